@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the single-GPU simulation path on MI355X.
+
+Workload (default): Brunel + E->E STDP at 1e9 synapses -- three populations (Poisson 20 %,
+excitatory, inhibitory; N = 100,000), every population pair connected at p = 0.1, delay 15,
+dt = 0.1 ms, additive pair STDP on excitatory->excitatory synapses (model brunel_3pop_stdp).
+Synthetic by construction: the random graph is generated on the device from a fixed seed and
+all state starts from the model's init.
+
+One "step" is one simulation time step of the whole network (spike delivery incl. lazy
+plasticity, neuron update, plasticity flush). The timed region replays captured HIP graphs
+of steps with all state resident in HBM; value = wall seconds / simulated (bio) seconds.
+
+Printed: one JSON line with the metric, synapse events/s, a roofline object for the
+dominant kernel (algorithmic bytes per launch / mean launch time from HIP events on the
+net's stream) and a cpu_baseline object (the single-threaded CPU checker, a restatement of
+the reference's cpu::snn, timed on this host on a bounded number of steps of the same net).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "wall/bio ratio @1e9 syn; batched syn-events/s; p50/p99 job latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def three_pop(n, p):
+    npois = n // 5
+    ne = (n - npois) * 4 // 5
+    return [npois, ne, n - npois - ne], [(s, d, p) for s in range(3) for d in range(3)]
+
+
+WORKLOADS = {
+    # name: (model, n, p, delay)
+    "brunel_stdp_1e9": ("brunel_3pop_stdp", 100_000, 0.1, 15),
+    "brunel_1e9": ("brunel_3pop", 100_000, 0.1, 15),
+    "vogels_1e8": ("vogels_abbott", 70_710, 0.02, 1),
+}
+
+
+def algorithmic_bytes(model, n, st, steps):
+    """Algorithmic HBM bytes per launch of each kernel, from the device counters of the
+    profiled steps. Per delivered spike: its id (4) + row offsets (16); per delivered event:
+    target id (4) + synapse weight read (4, models with per-synapse state); per plastic
+    synapse brought up to date: read + write of its 12-byte state minus the weight read
+    already counted (20). Neuron update: state read + write, accumulator read + clear, and for
+    plastic models history (16) and row bookkeeping (9) per neuron. Flush: target ids of the
+    flushed rows' plastic synapses (4) + their 12-byte state read + write (24). Target-neuron
+    accumulation is counted as atomics, not bytes."""
+    from spice_amd.models import MODELS
+    _, nattrs, sattrs = MODELS[model]
+    s_n = sum(np.dtype(t).itemsize for _, t in nattrs)
+    plastic = bool(sattrs)
+    acc_bytes = 4 * (1 if "brunel" in model else 2)  # receive-target attributes per neuron
+    spk = st["spikes"] / steps
+    ev = st["events"] / steps
+    pev = st["plastic_events"] / steps
+    recv = spk * 20 + ev * (8 if plastic else 4) + pev * (20 if plastic else 0)
+    upd = n * (2 * s_n + 2 * acc_bytes + (25 if plastic else 0)) + spk * 4
+    fl = (st["flush_synapses"] / steps) * 28 if plastic else 0.0
+    return {"receive": recv, "update": upd, "flush": fl}
+
+
+def cpu_baseline(model, lay, delay, seed, net, steps):
+    """Times the CPU checker (single thread) on `steps` steps of the same network: the graph
+    is copied from the GPU, state starts from the model's init (as the GPU run did)."""
+    from oracle import oracle as O
+    rp, col = net.csr()
+    t0 = time.perf_counter()
+    h = O.CpuSnn.from_csr(model, rp, col, net.width(), 1e-4, delay, seed=seed)
+    t_init = time.perf_counter() - t0
+    del rp, col
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.step()
+    dt_wall = time.perf_counter() - t0
+    h.close()
+    return dt_wall / steps, t_init
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10_000, help="timed steps (10,000 = 1 s bio)")
+    ap.add_argument("--warmup", type=int, default=2_000)
+    ap.add_argument("--workload", default="brunel_stdp_1e9", choices=sorted(WORKLOADS))
+    ap.add_argument("--profile-steps", type=int, default=1_000)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline (0 = skip)")
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU path is a design note, not an implementation", file=sys.stderr)
+        sys.exit(2)
+
+    from spice_amd import Layout, Snn
+
+    model, n, p, delay = WORKLOADS[args.workload]
+    pops, conns = three_pop(n, p) if "brunel" in model else ([n], [(0, 0, p)])
+    dt = 1e-4
+
+    t0 = time.perf_counter()
+    net = Snn(model, Layout(pops, conns), dt, delay, seed=args.seed)
+    net.synchronize()
+    setup_s = time.perf_counter() - t0
+    E = net.num_edges()
+    print(f"[bench] {args.workload}: N={n} E={E} setup {setup_s:.3f}s", file=sys.stderr, flush=True)
+
+    net.run(args.warmup)
+    net.synchronize()
+    net.reset_stats()
+
+    t0 = time.perf_counter()
+    net.run(args.steps)
+    net.synchronize()
+    wall = time.perf_counter() - t0
+    st = net.stats()
+    bio = args.steps * dt
+    print(f"[bench] timed {args.steps} steps in {wall:.3f}s, stats {st}", file=sys.stderr, flush=True)
+
+    # Roofline: per-kernel mean launch time from HIP events on the net's stream.
+    net.reset_stats()
+    ms = net.run_profiled(args.profile_steps)
+    pst = net.stats()
+    bytes_per = algorithmic_bytes(model, n, pst, args.profile_steps)
+    kern = max(ms, key=lambda k: ms[k])
+    avg_ms = ms[kern] / args.profile_steps
+    achieved = bytes_per[kern] / (avg_ms * 1e-3) / 1e9
+    kernel_names = {"receive": "k_receive", "update": "k_update", "flush": "k_flush"}
+    roofline = {
+        "bound": "hbm", "kernel": kernel_names[kern], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+        "bytes_per_launch": round(bytes_per[kern]), "avg_launch_us": round(avg_ms * 1e3, 3),
+        "kernel_us_per_step": {k: round(v / args.profile_steps * 1e3, 3) for k, v in ms.items()},
+    }
+
+    cpu = None
+    if args.cpu_steps > 0:
+        s_per_step, t_init = cpu_baseline(model, (pops, conns), delay, args.seed, net, args.cpu_steps)
+        cpu = {"value": round(s_per_step / dt, 3), "unit": "wall s / bio s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same {E:.3g}-synapse network (graph copied from the GPU, "
+                         f"state from init; dense per-step plasticity as in cpu::snn), "
+                         f"{s_per_step:.2f} s/step, wall/bio extrapolated linearly; init {t_init:.1f}s excluded"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(wall / bio, 5),
+        "unit": "wall s / bio s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-generated random graph, fixed seed)",
+        "config": {"workload": args.workload, "model": model, "neurons": n, "synapses": E,
+                   "populations": pops, "p": p, "delay": delay, "dt": dt, "bio_s": bio},
+        "syn_events_per_s": round(st["events"] / wall, 1),
+        "spikes_per_step": round(st["spikes"] / args.steps, 2),
+        "events_per_step": round(st["events"] / args.steps, 1),
+        "setup_s": round(setup_s, 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    net.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,106 @@
+#pragma once
+// The model plug-in contract.
+//
+// A model is a struct with a nested `neuron` type and, optionally, a nested `synapse`
+// type. Each derives from neuron<Ts...> / synapse<Ts...>, which declares the per-element
+// state attributes, and overrides any of these static functors (same contract as the
+// reference, spice/models/model.h:10-58, so reference models compile unchanged):
+//
+//   neuron::init(it, info, backend)                    once per neuron
+//   neuron::update(it, dt, info, backend) -> bool      once per neuron per step; true = spike
+//   neuron::receive(src, dst_it, syn_it, info, backend) once per delivered synaptic event
+//   synapse::init(syn_it, src, dst, info, backend)     once per synapse
+//   synapse::update(syn_it, nsteps, pre, post, dt, info, backend)
+//                                                      plasticity over nsteps steps; the
+//                                                      last step carries the pre/post flags
+//   synapse::plastic(src, dst, info) -> bool           which synapses update() applies to
+//
+// The backend offers rand() in [0, 1), atomic_add(target, value), min, max, clamp, exp, pow.
+//
+// Contract points that a parallel implementation fixes (and the CPU checker follows):
+//  * receive() may only modify neuron state through backend.atomic_add, and reads of the
+//    target neuron see its state as of the end of the previous step (inputs delivered in
+//    the same step are not visible to each other);
+//  * synapse::update(n, pre, post) must equal n-1 event-free steps followed by one step
+//    with the given flags (this is what lets plasticity be applied lazily).
+//
+// Optional trait: a neuron type may declare
+//     static constexpr std::uint32_t receive_mask = (1u << A) | (1u << B);
+// naming the attributes receive() adds to. Without it every attribute gets an input
+// accumulator.
+#include <spice/snn_info.h>
+#include <spice/util/host_defines.h>
+#include <spice/util/meta.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace spice
+{
+template <typename... Ts>
+struct neuron : util::type_list<Ts...>
+{
+	template <typename Iter, typename Backend>
+	HYBRID static void init( Iter, snn_info, Backend & )
+	{
+	}
+
+	template <typename Iter, typename Backend>
+	HYBRID static bool update( Iter, float, snn_info, Backend & )
+	{
+		return false;
+	}
+
+	template <typename Iter, typename SynIter, typename Backend>
+	HYBRID static void receive( int_, Iter, SynIter &&, snn_info, Backend & )
+	{
+	}
+};
+
+template <typename... Ts>
+struct synapse : util::type_list<Ts...>
+{
+	template <typename Iter, typename Backend>
+	HYBRID static void init( Iter, int_, int_, snn_info, Backend & )
+	{
+	}
+
+	template <typename Iter, typename Backend>
+	HYBRID static void update( Iter &&, int_, bool, bool, float, snn_info, Backend & )
+	{
+	}
+
+	HYBRID static bool plastic( int_, int_, snn_info ) { return true; }
+};
+
+struct model
+{
+	struct neuron : ::spice::neuron<>
+	{
+	};
+	struct synapse : ::spice::synapse<>
+	{
+	};
+};
+
+namespace detail
+{
+template <typename N, typename = void>
+struct receive_mask_of
+{
+	static constexpr std::uint32_t value = N::size >= 32 ? 0xffffffffu : ( ( 1u << N::size ) - 1u );
+};
+template <typename N>
+struct receive_mask_of<N, std::void_t<decltype( N::receive_mask )>>
+{
+	static constexpr std::uint32_t value = N::receive_mask;
+};
+} // namespace detail
+
+// Bit I set <=> neuron attribute I can be the target of receive()'s atomic_add.
+template <typename Neuron>
+inline constexpr std::uint32_t receive_mask_v = detail::receive_mask_of<Neuron>::value;
+
+template <typename Model>
+inline constexpr bool is_plastic_v = Model::synapse::size > 0;
+} // namespace spice

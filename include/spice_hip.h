@@ -1,0 +1,249 @@
+/*
+ * spice_hip.h - C ABI of the MI355X (gfx950) single-GPU spiking-network simulator.
+ *
+ * This is the drop-in boundary beside the reference's cuda::snn<Model>. The reference's
+ * backend is a C++ template (spice/cuda/snn.h:16-77) that drives a templated kernel layer
+ * (spice/cuda/algorithm.h:15-72) explicitly instantiated per model
+ * (spice/cuda/algorithm.cu:490-747). A C ABI cannot carry templates, so the model is a
+ * run-time id, the network is an opaque handle that owns every device buffer, and all
+ * arguments are plain pointers and sizes. include/spice/hip/snn.h wraps this ABI back into
+ * the reference's C++ API (spice::snn<Model>).
+ *
+ * Errors: every function returns a spice_status. On failure spice_last_error() returns a
+ * thread-local message; the C++ wrapper rethrows the reference's exception types
+ * (std::invalid_argument, std::bad_alloc, std::runtime_error, std::bad_cast).
+ *
+ * Threading: a net is not thread-safe; different nets may be driven from different host
+ * threads. All work of one net goes to the net's own HIP stream and is asynchronous unless
+ * stated otherwise.
+ */
+#ifndef SPICE_HIP_H
+#define SPICE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Models compiled into the library. 0-3 are the reference's models
+ * (spice/models/{vogels_abbott,brunel,brunel_with_plasticity,synth}.h); 4-5 are the
+ * three-population Brunel variants of the benchmark configurations. */
+enum spice_model_id
+{
+	SPICE_MODEL_VOGELS_ABBOTT = 0,
+	SPICE_MODEL_BRUNEL = 1,
+	SPICE_MODEL_BRUNEL_WITH_PLASTICITY = 2,
+	SPICE_MODEL_SYNTH = 3,
+	SPICE_MODEL_BRUNEL_3POP = 4,
+	SPICE_MODEL_BRUNEL_3POP_STDP = 5,
+	SPICE_NUM_MODELS = 6
+};
+
+enum spice_status
+{
+	SPICE_OK = 0,
+	SPICE_ERR_INVALID_ARGUMENT = 1, /* reference: spice_assert -> std::invalid_argument (spice/util/assert.cpp:22-35) */
+	SPICE_ERR_BAD_ALLOC = 2,        /* reference: cuda_malloc -> std::bad_alloc (spice/cuda/util/memory.cpp:9-15) */
+	SPICE_ERR_RUNTIME = 3,          /* reference: success_or_throw -> std::runtime_error (spice/cuda/util/error.h:19-53) */
+	SPICE_ERR_BAD_CAST = 4          /* reference: narrow -> std::bad_cast (spice/util/type_traits.h:115) */
+};
+
+enum spice_flag
+{
+	/* Sum receive() contributions per target attribute as int64 fixed point (units of
+	 * 2^-40) instead of fp32 atomics. Integer addition is associative, so results do not
+	 * depend on atomic order and match the CPU checker's fixed-point mode bit for bit. */
+	SPICE_FLAG_FIXED_POINT = 1u << 0
+};
+
+/* Attribute type codes reported by spice_model_info. */
+enum spice_attr_type
+{
+	SPICE_ATTR_F32 = 0,
+	SPICE_ATTR_I32 = 1
+};
+
+typedef struct spice_net spice_net;
+
+/* (src population, dst population, connection probability): one entry of the reference's
+ * layout connectivity list (spice/util/layout.h:31-33). */
+typedef struct
+{
+	uint64_t src_pop;
+	uint64_t dst_pop;
+	float p;
+} spice_connection;
+
+/* Flattened connection ([src_first, src_last), [dst_first, dst_last), p): the reference's
+ * layout::edge (spice/util/layout.h:27-28). */
+typedef struct
+{
+	int32_t src_first, src_last, dst_first, dst_last;
+	float p;
+} spice_edge;
+
+/* Counters accumulated on the device since creation (or the last reset). */
+typedef struct
+{
+	uint64_t steps;          /* completed steps */
+	uint64_t spikes;         /* spikes emitted */
+	uint64_t events;         /* synaptic events delivered (real edges of delivered spikes) */
+	uint64_t plastic_events; /* plastic synapses brought up to date on spike delivery */
+	uint64_t flush_rows;     /* rows flushed because their post-spike history window filled */
+	uint64_t flush_synapses; /* plastic synapses touched by those flushes */
+} spice_stats;
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char * spice_last_error( void );
+
+/* Library build string (compiler, target, flags). */
+const char * spice_version( void );
+
+/* ---- host utilities ------------------------------------------------------------------- */
+
+/* Validates and flattens a population/connection description exactly as the reference's
+ * layout constructor does (spice/util/layout.cpp:51-110). out_edges must hold nconns
+ * entries. Replaces util::layout::{layout, connections, size, max_degree}. */
+int spice_layout_flatten(
+    const uint64_t * pops,
+    int32_t npops,
+    const spice_connection * conns,
+    int32_t nconns,
+    spice_edge * out_edges,
+    int32_t * out_nedges,
+    uint64_t * out_num_neurons,
+    uint64_t * out_max_degree );
+
+/* Number and types of a model's neuron/synapse attributes (the reference's
+ * Model::neuron::tuple_t / Model::synapse::tuple_t, spice/util/meta.h:130-171). The type
+ * arrays must hold 16 entries. */
+int spice_model_info(
+    int32_t model,
+    int32_t * n_neuron_attrs,
+    int32_t * n_synapse_attrs,
+    int32_t * neuron_attr_types,
+    int32_t * synapse_attr_types );
+
+/* ---- construction --------------------------------------------------------------------- */
+
+/* Replaces cuda::snn<Model>::snn(layout const&, float dt, int delay) (spice/cuda/snn.h:20-25,
+ * spice/cuda/snn.d.cpp:58-87): builds the random graph on the device (Philox generator keyed
+ * by `seed`), allocates and initialises state. The graph is CSR with rows clipped to the
+ * layout's max_degree, so it exports to the reference's ELL format unchanged. */
+int spice_hip_create(
+    int32_t model,
+    const uint64_t * pops,
+    int32_t npops,
+    const spice_connection * conns,
+    int32_t nconns,
+    float dt,
+    int32_t delay,
+    uint64_t seed,
+    uint32_t flags,
+    spice_net ** out_net );
+
+/* Replaces cuda::snn<Model>::snn(std::vector<int> const& adj, size_t width, float dt, int delay)
+ * (spice/cuda/snn.h:26-32, spice/cuda/snn.d.cpp:89-121): imports an ELL adjacency
+ * (num_neurons x width, each row sorted, padded with INT_MAX). width must be a multiple of 32. */
+int spice_hip_create_from_ell(
+    int32_t model,
+    const int32_t * edges,
+    uint64_t num_neurons,
+    uint64_t width,
+    float dt,
+    int32_t delay,
+    uint64_t seed,
+    uint32_t flags,
+    spice_net ** out_net );
+
+/* Same as above from a CSR adjacency (row_ptr has num_neurons + 1 entries); `width` is the
+ * ELL width reported by num_synapses()/adj() and must be >= every row length. */
+int spice_hip_create_from_csr(
+    int32_t model,
+    const int64_t * row_ptr,
+    const int32_t * col,
+    uint64_t num_neurons,
+    uint64_t width,
+    float dt,
+    int32_t delay,
+    uint64_t seed,
+    uint32_t flags,
+    spice_net ** out_net );
+
+/* Releases every device buffer of the net (the reference's dbuffer RAII,
+ * spice/cuda/util/dbuffer.h:17-90). */
+int spice_hip_destroy( spice_net * net );
+
+/* ---- simulation ------------------------------------------------------------------------ */
+
+/* Replaces cuda::snn<Model>::step(int** out_dspikes, uint** out_dnum_spikes, ...)
+ * (spice/cuda/snn.h:36-37, spice/cuda/snn.d.cpp:233-289): enqueues one step. The returned
+ * device pointers (either may be NULL) address this step's spike ids and spike count; they
+ * stay valid until the next step is enqueued. Asynchronous. */
+int spice_hip_step( spice_net * net, int32_t ** out_dspikes, uint32_t ** out_dcount );
+
+/* Copies the spike ids of the most recent step to host memory (synchronous), as
+ * cuda::snn<Model>::step(std::vector<int>*) does (spice/cuda/snn.d.cpp:280-287).
+ * Spike order is unspecified (atomic append), as in the reference. */
+int spice_hip_copy_spikes( spice_net * net, int32_t * out, uint64_t cap, uint64_t * out_count );
+
+/* Enqueues nsteps steps without spike readback, replaying a captured HIP graph of the
+ * step sequence. Equivalent to nsteps calls of spice_hip_step. Asynchronous. */
+int spice_hip_run( spice_net * net, int64_t nsteps );
+
+/* Enqueues nsteps steps without graph capture, with HIP events recorded around every kernel
+ * on the net's stream, waits, and adds the summed per-kernel device times in milliseconds to
+ * kernel_ms[0] (spike delivery), kernel_ms[1] (neuron update) and kernel_ms[2] (plasticity
+ * flush; plastic models only). Same results as spice_hip_run. Used for roofline figures. */
+int spice_hip_run_profiled( spice_net * net, int64_t nsteps, double * kernel_ms );
+
+/* Waits for all work of the net. */
+int spice_hip_synchronize( spice_net * net );
+
+/* ---- queries ---------------------------------------------------------------------------- */
+
+/* num_neurons() / num_synapses() (= num_neurons x ELL width, the reference's count,
+ * spice/util/adj_list.cpp:97) / ELL width / number of real edges / dt() / delay()
+ * (spice/snn.h:24-28). */
+int spice_hip_num_neurons( spice_net * net, uint64_t * out );
+int spice_hip_num_synapses( spice_net * net, uint64_t * out );
+int spice_hip_width( spice_net * net, uint64_t * out );
+int spice_hip_num_edges( spice_net * net, uint64_t * out );
+int spice_hip_dt( spice_net * net, float * out );
+int spice_hip_delay( spice_net * net, int32_t * out );
+int spice_hip_model( spice_net * net, int32_t * out );
+
+/* Step counter and Kahan-summed simulated time (sum, compensation) of the base class
+ * spice::snn (spice/snn.h:40-44, spice/util/numeric.h:8-33). */
+int spice_hip_get_clock( spice_net * net, int64_t * step, float * sum, float * c );
+int spice_hip_set_clock( spice_net * net, int64_t step, float sum, float c );
+
+int spice_hip_get_stats( spice_net * net, spice_stats * out );
+int spice_hip_reset_stats( spice_net * net );
+
+/* ---- state export / import (synchronous) ------------------------------------------------ */
+
+/* adj(): ELL edges (num_neurons x width, INT_MAX-padded) (spice/snn.h:32,
+ * spice/cuda/snn.d.cpp:308-312). `cap` is in elements. */
+int spice_hip_export_adj( spice_net * net, int32_t * out, uint64_t cap );
+
+/* CSR adjacency: row_ptr (num_neurons + 1) and col (num_edges). */
+int spice_hip_export_csr( spice_net * net, int64_t * row_ptr, int32_t * col );
+
+/* neurons(): one attribute at a time, num_neurons elements (spice/snn.h:33,
+ * spice/util/meta.h:212-233). `bytes` must equal num_neurons * attribute size. */
+int spice_hip_export_neuron_attr( spice_net * net, int32_t attr, void * out, uint64_t bytes );
+int spice_hip_import_neuron_attr( spice_net * net, int32_t attr, const void * in, uint64_t bytes );
+
+/* synapses(): one attribute at a time. ell_order != 0: num_neurons x width slots in the
+ * reference's ELL slot order (src * width + j, zero padding) (spice/snn.h:34); ell_order == 0:
+ * num_edges entries in CSR order. Plastic synapses are brought up to date first. */
+int spice_hip_export_synapse_attr( spice_net * net, int32_t attr, void * out, uint64_t bytes, int32_t ell_order );
+int spice_hip_import_synapse_attr( spice_net * net, int32_t attr, const void * in, uint64_t bytes, int32_t ell_order );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPICE_HIP_H */

@@ -1,0 +1,114 @@
+"""ctypes binding of the gfx950 simulator's C ABI (include/spice_hip.h).
+
+The library is built in-tree (spice_amd/libspice_hip.so, see __graft_entry__.build or
+`make -C spice_amd/csrc`). There is no fallback: if it is missing, importing the binding
+raises, so nothing can silently run anywhere but on the GPU.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspice_hip.so")
+
+SPICE_OK = 0
+SPICE_ERR_INVALID_ARGUMENT = 1
+SPICE_ERR_BAD_ALLOC = 2
+SPICE_ERR_RUNTIME = 3
+SPICE_ERR_BAD_CAST = 4
+SPICE_FLAG_FIXED_POINT = 1
+
+
+class SpiceError(RuntimeError):
+    """Raised for SPICE_ERR_RUNTIME (the reference's std::runtime_error)."""
+
+
+class SpiceCastError(TypeError):
+    """Raised for SPICE_ERR_BAD_CAST (the reference's std::bad_cast)."""
+
+
+class Connection(ctypes.Structure):
+    _fields_ = [("src_pop", ctypes.c_uint64), ("dst_pop", ctypes.c_uint64), ("p", ctypes.c_float)]
+
+
+class Edge(ctypes.Structure):
+    _fields_ = [("src_first", ctypes.c_int32), ("src_last", ctypes.c_int32),
+                ("dst_first", ctypes.c_int32), ("dst_last", ctypes.c_int32), ("p", ctypes.c_float)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("steps", "spikes", "events", "plastic_events", "flush_rows", "flush_synapses")]
+
+
+_P = ctypes.c_void_p
+_I32, _U32, _I64, _U64, _F32 = ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
+_PI32, _PU32, _PI64, _PU64, _PF32 = (ctypes.POINTER(t) for t in (_I32, _U32, _I64, _U64, _F32))
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "spice_last_error": (ctypes.c_char_p, []),
+    "spice_version": (ctypes.c_char_p, []),
+    "spice_layout_flatten": (ctypes.c_int, [_PU64, _I32, ctypes.POINTER(Connection), _I32,
+                                            ctypes.POINTER(Edge), _PI32, _PU64, _PU64]),
+    "spice_model_info": (ctypes.c_int, [_I32, _PI32, _PI32, _PI32, _PI32]),
+    "spice_hip_create": (ctypes.c_int, [_I32, _PU64, _I32, ctypes.POINTER(Connection), _I32, _F32, _I32,
+                                        _U64, _U32, ctypes.POINTER(_P)]),
+    "spice_hip_create_from_ell": (ctypes.c_int, [_I32, _PI32, _U64, _U64, _F32, _I32, _U64, _U32,
+                                                 ctypes.POINTER(_P)]),
+    "spice_hip_create_from_csr": (ctypes.c_int, [_I32, _PI64, _PI32, _U64, _U64, _F32, _I32, _U64, _U32,
+                                                 ctypes.POINTER(_P)]),
+    "spice_hip_destroy": (ctypes.c_int, [_P]),
+    "spice_hip_step": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "spice_hip_copy_spikes": (ctypes.c_int, [_P, _PI32, _U64, _PU64]),
+    "spice_hip_run": (ctypes.c_int, [_P, _I64]),
+    "spice_hip_run_profiled": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_double)]),
+    "spice_hip_synchronize": (ctypes.c_int, [_P]),
+    "spice_hip_num_neurons": (ctypes.c_int, [_P, _PU64]),
+    "spice_hip_num_synapses": (ctypes.c_int, [_P, _PU64]),
+    "spice_hip_width": (ctypes.c_int, [_P, _PU64]),
+    "spice_hip_num_edges": (ctypes.c_int, [_P, _PU64]),
+    "spice_hip_dt": (ctypes.c_int, [_P, _PF32]),
+    "spice_hip_delay": (ctypes.c_int, [_P, _PI32]),
+    "spice_hip_model": (ctypes.c_int, [_P, _PI32]),
+    "spice_hip_get_clock": (ctypes.c_int, [_P, _PI64, _PF32, _PF32]),
+    "spice_hip_set_clock": (ctypes.c_int, [_P, _I64, _F32, _F32]),
+    "spice_hip_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    "spice_hip_reset_stats": (ctypes.c_int, [_P]),
+    "spice_hip_export_adj": (ctypes.c_int, [_P, _PI32, _U64]),
+    "spice_hip_export_csr": (ctypes.c_int, [_P, _PI64, _PI32]),
+    "spice_hip_export_neuron_attr": (ctypes.c_int, [_P, _I32, _P, _U64]),
+    "spice_hip_import_neuron_attr": (ctypes.c_int, [_P, _I32, _P, _U64]),
+    "spice_hip_export_synapse_attr": (ctypes.c_int, [_P, _I32, _P, _U64, _I32]),
+    "spice_hip_import_synapse_attr": (ctypes.c_int, [_P, _I32, _P, _U64, _I32]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libspice_hip.so (once). Raises OSError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} not built: run `make -C spice_amd/csrc` or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    """Maps a spice_status to the Python counterpart of the reference's exception types."""
+    if status == SPICE_OK:
+        return
+    msg = lib().spice_last_error().decode(errors="replace")
+    if status == SPICE_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if status == SPICE_ERR_BAD_ALLOC:
+        raise MemoryError(msg)
+    if status == SPICE_ERR_BAD_CAST:
+        raise SpiceCastError(msg)
+    raise SpiceError(msg)

@@ -1,0 +1,394 @@
+// C ABI of the gfx950 simulator (declared in include/spice_hip.h).
+#include "graph.h"
+#include "net.h"
+
+#include <spice/models/all.h>
+#include <spice/util/layout.h>
+#include <spice_hip.h>
+
+#include <climits>
+#include <cstring>
+#include <new>
+#include <string>
+#include <typeinfo>
+
+namespace spice::hipx
+{
+net_iface * make_net_vogels_abbott( net_params const &, csr_dev &&, hipStream_t );
+net_iface * make_net_brunel( net_params const &, csr_dev &&, hipStream_t );
+net_iface * make_net_brunel_with_plasticity( net_params const &, csr_dev &&, hipStream_t );
+net_iface * make_net_synth( net_params const &, csr_dev &&, hipStream_t );
+net_iface * make_net_brunel_3pop( net_params const &, csr_dev &&, hipStream_t );
+net_iface * make_net_brunel_3pop_stdp( net_params const &, csr_dev &&, hipStream_t );
+} // namespace spice::hipx
+
+using namespace spice;
+using namespace spice::hipx;
+
+struct spice_net
+{
+	net_iface * impl;
+};
+
+static thread_local std::string g_err;
+
+template <typename F>
+static int guard( F && f )
+{
+	try
+	{
+		g_err.clear();
+		f();
+		return SPICE_OK;
+	}
+	catch( std::invalid_argument const & e )
+	{
+		g_err = e.what();
+		return SPICE_ERR_INVALID_ARGUMENT;
+	}
+	catch( std::bad_alloc const & e )
+	{
+		g_err = std::string( "bad_alloc: " ) + e.what();
+		return SPICE_ERR_BAD_ALLOC;
+	}
+	catch( std::bad_cast const & e )
+	{
+		g_err = e.what();
+		return SPICE_ERR_BAD_CAST;
+	}
+	catch( std::exception const & e )
+	{
+		g_err = e.what();
+		return SPICE_ERR_RUNTIME;
+	}
+}
+
+static void require( bool c, char const * msg )
+{
+	if( !c ) throw std::invalid_argument( msg );
+}
+
+static util::layout make_layout( uint64_t const * pops, int32_t npops, spice_connection const * conns, int32_t nconns )
+{
+	require( npops >= 0 && nconns >= 0 && ( pops || npops == 0 ) && ( conns || nconns == 0 ), "invalid layout arguments" );
+	std::vector<size_t> p( pops, pops + npops );
+	std::vector<std::tuple<size_t, size_t, float>> c;
+	for( int32_t i = 0; i < nconns; i++ ) c.emplace_back( conns[i].src_pop, conns[i].dst_pop, conns[i].p );
+	return util::layout( p, c );
+}
+
+static std::vector<util::gen::conn> gen_conns( util::layout const & l )
+{
+	std::vector<util::gen::conn> out;
+	for( auto const & c : l.connections() )
+	{
+		util::gen::conn g;
+		g.src_first = std::get<0>( c );
+		g.src_last = std::get<1>( c );
+		g.dst_first = std::get<2>( c );
+		g.dst_last = std::get<3>( c );
+		double const range = double( g.dst_last - g.dst_first );
+		double const p = std::get<4>( c );
+		g.mean = range * p;
+		g.sd = std::sqrt( range * p * ( 1.0 - p ) );
+		out.push_back( g );
+	}
+	return out;
+}
+
+static hipStream_t new_stream()
+{
+	hipStream_t s = nullptr;
+	SPICE_HIP( hipStreamCreateWithFlags( &s, hipStreamNonBlocking ) );
+	return s;
+}
+
+static net_iface * make_net( net_params const & p, csr_dev && g, hipStream_t s )
+{
+	try
+	{
+		switch( p.model )
+		{
+		case SPICE_MODEL_VOGELS_ABBOTT: return make_net_vogels_abbott( p, std::move( g ), s );
+		case SPICE_MODEL_BRUNEL: return make_net_brunel( p, std::move( g ), s );
+		case SPICE_MODEL_BRUNEL_WITH_PLASTICITY: return make_net_brunel_with_plasticity( p, std::move( g ), s );
+		case SPICE_MODEL_SYNTH: return make_net_synth( p, std::move( g ), s );
+		case SPICE_MODEL_BRUNEL_3POP: return make_net_brunel_3pop( p, std::move( g ), s );
+		case SPICE_MODEL_BRUNEL_3POP_STDP: return make_net_brunel_3pop_stdp( p, std::move( g ), s );
+		default: throw std::invalid_argument( "unknown model id" );
+		}
+	}
+	catch( ... )
+	{
+		// the net owns the stream only once constructed
+		(void)hipStreamDestroy( s );
+		throw;
+	}
+}
+
+static net_iface * impl( spice_net * n )
+{
+	require( n && n->impl, "null net handle" );
+	return n->impl;
+}
+
+extern "C" {
+const char * spice_last_error( void ) { return g_err.c_str(); }
+
+const char * spice_version( void )
+{
+	return "spice_hip gfx950 (hipcc " __VERSION__ ", -O3 -ffp-contract=off, wave64)";
+}
+
+int spice_layout_flatten(
+    uint64_t const * pops, int32_t npops, spice_connection const * conns, int32_t nconns, spice_edge * out, int32_t * nout, uint64_t * n, uint64_t * maxdeg )
+{
+	return guard( [&] {
+		require( out || nconns == 0, "null output" );
+		auto l = make_layout( pops, npops, conns, nconns );
+		int32_t i = 0;
+		for( auto const & e : l.connections() )
+			out[i++] = { std::get<0>( e ), std::get<1>( e ), std::get<2>( e ), std::get<3>( e ), std::get<4>( e ) };
+		if( nout ) *nout = i;
+		if( n ) *n = l.size();
+		if( maxdeg ) *maxdeg = l.max_degree();
+	} );
+}
+
+int spice_model_info( int32_t model, int32_t * nna, int32_t * nsa, int32_t * ntypes, int32_t * stypes )
+{
+	return guard( [&] {
+		dispatch_model( model, [&]( auto m ) {
+			using M = decltype( m );
+			using Nn = typename M::neuron;
+			using Ss = typename M::synapse;
+			if( nna ) *nna = int32_t( Nn::size );
+			if( nsa ) *nsa = int32_t( Ss::size );
+			util::for_n<Nn::size>( [&]( auto I ) {
+				if( ntypes ) ntypes[I] = std::is_floating_point_v<typename Nn::template type<I>> ? SPICE_ATTR_F32 : SPICE_ATTR_I32;
+			} );
+			util::for_n<Ss::size>( [&]( auto I ) {
+				if( stypes ) stypes[I] = std::is_floating_point_v<typename Ss::template type<I>> ? SPICE_ATTR_F32 : SPICE_ATTR_I32;
+			} );
+			return 0;
+		} );
+	} );
+}
+
+int spice_hip_create(
+    int32_t model,
+    uint64_t const * pops,
+    int32_t npops,
+    spice_connection const * conns,
+    int32_t nconns,
+    float dt,
+    int32_t delay,
+    uint64_t seed,
+    uint32_t flags,
+    spice_net ** out )
+{
+	return guard( [&] {
+		require( out != nullptr, "null output handle" );
+		auto l = make_layout( pops, npops, conns, nconns );
+		require( dt > 0.0f, "dt must be positive" );
+		require( delay >= 1, "delay must be >= 1" );
+		hipStream_t s = new_stream();
+		csr_dev g;
+		try
+		{
+			g = generate_csr( gen_conns( l ), int64_t( l.size() ), int64_t( l.max_degree() ), seed, s );
+		}
+		catch( ... )
+		{
+			(void)hipStreamDestroy( s );
+			throw;
+		}
+		*out = new spice_net{ make_net( { model, dt, delay, seed, flags }, std::move( g ), s ) };
+	} );
+}
+
+int spice_hip_create_from_csr(
+    int32_t model,
+    int64_t const * row_ptr,
+    int32_t const * col,
+    uint64_t n,
+    uint64_t width,
+    float dt,
+    int32_t delay,
+    uint64_t seed,
+    uint32_t flags,
+    spice_net ** out )
+{
+	return guard( [&] {
+		require( out != nullptr, "null output handle" );
+		require( row_ptr != nullptr && n > 0 && n < uint64_t( INT_MAX ), "invalid node count" );
+		require( row_ptr[0] == 0, "row_ptr[0] must be 0" );
+		for( uint64_t r = 0; r < n; r++ )
+		{
+			int64_t const len = row_ptr[r + 1] - row_ptr[r];
+			require( len >= 0 && uint64_t( len ) <= width, "row longer than the adjacency width" );
+		}
+		require( col != nullptr || row_ptr[n] == 0, "null column array" );
+		for( int64_t e = 0; e < row_ptr[n]; e++ ) require( col[e] >= 0 && uint64_t( col[e] ) < n, "neighbor id out of range" );
+		require( dt > 0.0f && delay >= 1, "dt must be positive and delay >= 1" );
+		hipStream_t s = new_stream();
+		csr_dev g;
+		try
+		{
+			g = upload_csr( row_ptr, col, int64_t( n ), int64_t( width ), s );
+		}
+		catch( ... )
+		{
+			(void)hipStreamDestroy( s );
+			throw;
+		}
+		*out = new spice_net{ make_net( { model, dt, delay, seed, flags }, std::move( g ), s ) };
+	} );
+}
+
+int spice_hip_create_from_ell(
+    int32_t model, int32_t const * edges, uint64_t n, uint64_t width, float dt, int32_t delay, uint64_t seed, uint32_t flags, spice_net ** out )
+{
+	std::vector<int64_t> rp;
+	std::vector<int32_t> col;
+	int const st = guard( [&] {
+		require( width > 0, "width must be positive" );
+		require( width % 32 == 0, "width must be a multiple of 32" ); // reference: cuda/snn.d.cpp:103-107
+		require( edges != nullptr && n > 0, "invalid adjacency" );
+		rp.assign( n + 1, 0 );
+		for( uint64_t r = 0; r < n; r++ )
+		{
+			for( uint64_t j = 0; j < width && edges[r * width + j] != INT_MAX; j++ ) col.push_back( edges[r * width + j] );
+			rp[r + 1] = int64_t( col.size() );
+		}
+	} );
+	if( st != SPICE_OK ) return st;
+	return spice_hip_create_from_csr( model, rp.data(), col.data(), n, width, dt, delay, seed, flags, out );
+}
+
+int spice_hip_destroy( spice_net * net )
+{
+	return guard( [&] {
+		if( !net ) return;
+		delete net->impl;
+		delete net;
+	} );
+}
+
+int spice_hip_step( spice_net * net, int32_t ** dspk, uint32_t ** dcnt )
+{
+	return guard( [&] { impl( net )->step( dspk, dcnt ); } );
+}
+int spice_hip_copy_spikes( spice_net * net, int32_t * out, uint64_t cap, uint64_t * count )
+{
+	return guard( [&] {
+		require( count != nullptr, "null count" );
+		impl( net )->copy_spikes( out, cap, count );
+	} );
+}
+int spice_hip_run( spice_net * net, int64_t n )
+{
+	return guard( [&] { impl( net )->run( n ); } );
+}
+int spice_hip_run_profiled( spice_net * net, int64_t n, double * kernel_ms )
+{
+	return guard( [&] {
+		require( kernel_ms != nullptr, "null output" );
+		impl( net )->run_profiled( n, kernel_ms );
+	} );
+}
+int spice_hip_synchronize( spice_net * net )
+{
+	return guard( [&] { impl( net )->synchronize(); } );
+}
+
+int spice_hip_num_neurons( spice_net * net, uint64_t * out )
+{
+	return guard( [&] { *out = uint64_t( impl( net )->num_neurons() ); } );
+}
+int spice_hip_num_synapses( spice_net * net, uint64_t * out )
+{
+	return guard( [&] { *out = uint64_t( impl( net )->num_neurons() * impl( net )->width() ); } );
+}
+int spice_hip_width( spice_net * net, uint64_t * out )
+{
+	return guard( [&] { *out = uint64_t( impl( net )->width() ); } );
+}
+int spice_hip_num_edges( spice_net * net, uint64_t * out )
+{
+	return guard( [&] { *out = uint64_t( impl( net )->num_edges() ); } );
+}
+int spice_hip_dt( spice_net * net, float * out )
+{
+	return guard( [&] { *out = impl( net )->params().dt; } );
+}
+int spice_hip_delay( spice_net * net, int32_t * out )
+{
+	return guard( [&] { *out = impl( net )->params().delay; } );
+}
+int spice_hip_model( spice_net * net, int32_t * out )
+{
+	return guard( [&] { *out = impl( net )->params().model; } );
+}
+int spice_hip_get_clock( spice_net * net, int64_t * step, float * sum, float * c )
+{
+	return guard( [&] { impl( net )->get_clock( step, sum, c ); } );
+}
+int spice_hip_set_clock( spice_net * net, int64_t step, float sum, float c )
+{
+	return guard( [&] { impl( net )->set_clock( step, sum, c ); } );
+}
+int spice_hip_get_stats( spice_net * net, spice_stats * out )
+{
+	return guard( [&] {
+		require( out != nullptr, "null output" );
+		impl( net )->get_stats( out );
+	} );
+}
+int spice_hip_reset_stats( spice_net * net )
+{
+	return guard( [&] { impl( net )->reset_stats(); } );
+}
+int spice_hip_export_adj( spice_net * net, int32_t * out, uint64_t cap )
+{
+	return guard( [&] {
+		require( out != nullptr, "null output" );
+		impl( net )->export_adj( out, cap );
+	} );
+}
+int spice_hip_export_csr( spice_net * net, int64_t * row_ptr, int32_t * col )
+{
+	return guard( [&] {
+		require( row_ptr != nullptr && ( col != nullptr || impl( net )->num_edges() == 0 ), "null output" );
+		impl( net )->export_csr( row_ptr, col );
+	} );
+}
+int spice_hip_export_neuron_attr( spice_net * net, int32_t a, void * out, uint64_t bytes )
+{
+	return guard( [&] {
+		require( out != nullptr, "null output" );
+		impl( net )->export_neuron_attr( a, out, bytes );
+	} );
+}
+int spice_hip_import_neuron_attr( spice_net * net, int32_t a, void const * in, uint64_t bytes )
+{
+	return guard( [&] {
+		require( in != nullptr, "null input" );
+		impl( net )->import_neuron_attr( a, in, bytes );
+	} );
+}
+int spice_hip_export_synapse_attr( spice_net * net, int32_t a, void * out, uint64_t bytes, int32_t ell )
+{
+	return guard( [&] {
+		require( out != nullptr || bytes == 0, "null output" );
+		impl( net )->export_synapse_attr( a, out, bytes, ell != 0 );
+	} );
+}
+int spice_hip_import_synapse_attr( spice_net * net, int32_t a, void const * in, uint64_t bytes, int32_t ell )
+{
+	return guard( [&] {
+		require( in != nullptr || bytes == 0, "null input" );
+		impl( net )->import_synapse_attr( a, in, bytes, ell != 0 );
+	} );
+}
+}

@@ -1,0 +1,527 @@
+#pragma once
+// Host side of one simulated network on one GPU: owns every device buffer, the HIP stream
+// and the captured step graph, and launches the kernels of kernels.h.
+//
+// Reference counterpart: cuda::snn<Model> (spice/cuda/snn.h:16-77, spice/cuda/snn.d.cpp).
+#include "common.h"
+#include "dview.h"
+#include "graph.h"
+#include "kernels.h"
+
+#include <spice/models/model.h>
+#include <spice/util/graphgen.h>
+#include <spice/util/numeric.h>
+#include <spice_hip.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace spice::hipx
+{
+struct net_params
+{
+	int model;
+	float dt;
+	int delay;
+	std::uint64_t seed;
+	std::uint32_t flags;
+};
+
+// Model-independent interface used by the C ABI.
+struct net_iface
+{
+	virtual ~net_iface() = default;
+	virtual void step( std::int32_t ** dspk, std::uint32_t ** dcnt ) = 0;
+	virtual void run( std::int64_t n ) = 0;
+	virtual void run_profiled( std::int64_t n, double * kernel_ms ) = 0;
+	virtual void copy_spikes( std::int32_t * out, std::uint64_t cap, std::uint64_t * count ) = 0;
+	virtual void synchronize() = 0;
+	virtual net_params const & params() const = 0;
+	virtual std::int64_t num_neurons() const = 0;
+	virtual std::int64_t width() const = 0;
+	virtual std::int64_t num_edges() const = 0;
+	virtual void get_clock( std::int64_t * step, float * sum, float * c ) const = 0;
+	virtual void set_clock( std::int64_t step, float sum, float c ) = 0;
+	virtual void get_stats( spice_stats * out ) = 0;
+	virtual void reset_stats() = 0;
+	virtual void export_csr( std::int64_t * row_ptr, std::int32_t * col ) = 0;
+	virtual void export_adj( std::int32_t * ell, std::uint64_t cap ) = 0;
+	virtual void export_neuron_attr( int a, void * out, std::uint64_t bytes ) = 0;
+	virtual void import_neuron_attr( int a, void const * in, std::uint64_t bytes ) = 0;
+	virtual void export_synapse_attr( int a, void * out, std::uint64_t bytes, bool ell ) = 0;
+	virtual void import_synapse_attr( int a, void const * in, std::uint64_t bytes, bool ell ) = 0;
+};
+
+inline int device_cu_count()
+{
+	static int cus = [] {
+		int dev = 0;
+		SPICE_HIP( hipGetDevice( &dev ) );
+		hipDeviceProp_t p;
+		SPICE_HIP( hipGetDeviceProperties( &p, dev ) );
+		return p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+	}();
+	return cus;
+}
+
+template <typename Model>
+class hip_net final : public net_iface
+{
+	using N = typename Model::neuron;
+	using S = typename Model::synapse;
+	static constexpr bool plastic = is_plastic_v<Model>;
+	static constexpr int graph_steps = 32; // steps per captured graph
+
+public:
+	hip_net( net_params const & p, csr_dev && g, hipStream_t stream )
+	    : _p( p )
+	    , _g( std::move( g ) )
+	    , _s( stream )
+	{
+		if( !( p.dt > 0.0f ) ) throw std::invalid_argument( "dt must be positive" );
+		if( p.delay < 1 ) throw std::invalid_argument( "delay must be >= 1" );
+		if( plastic && p.delay > HISTORY )
+			throw std::invalid_argument( "plastic models support delays of at most 64 steps" );
+		if( _g.n < 1 || _g.n > INT_MAX - 1024 ) throw std::invalid_argument( "invalid neuron count" );
+		static_assert( N::size <= MAXA && S::size <= MAXA, "at most 16 neuron/synapse attributes" );
+		allocate();
+		initialize();
+	}
+
+	~hip_net() override
+	{
+		if( _exec ) (void)hipGraphExecDestroy( _exec );
+		if( _s ) (void)hipStreamSynchronize( _s );
+		if( _s ) (void)hipStreamDestroy( _s );
+	}
+
+	// ---- simulation ----
+	void step( std::int32_t ** dspk, std::uint32_t ** dcnt ) override
+	{
+		launch_step();
+		std::int64_t const t = _step++;
+		_clock.add( _p.dt );
+		int const slot = int( t % ( _p.delay + 1 ) );
+		if( dspk ) *dspk = _ring.data() + std::int64_t( slot ) * _g.n;
+		if( dcnt ) *dcnt = _counts.data() + slot;
+	}
+
+	void run( std::int64_t n ) override
+	{
+		if( n < 0 ) throw std::invalid_argument( "negative step count" );
+		std::int64_t done = 0;
+		if( n >= graph_steps )
+		{
+			if( !_exec ) capture();
+			for( ; done + graph_steps <= n; done += graph_steps ) SPICE_HIP( hipGraphLaunch( _exec, _s ) );
+		}
+		for( ; done < n; done++ ) launch_step();
+		for( std::int64_t i = 0; i < n; i++ )
+		{
+			_step++;
+			_clock.add( _p.dt );
+		}
+	}
+
+	// Runs n steps eagerly with HIP events around every kernel on the net's stream and adds
+	// the summed kernel times (ms) to kernel_ms[0..2] = receive, update, flush.
+	void run_profiled( std::int64_t n, double * kernel_ms ) override
+	{
+		if( n < 0 ) throw std::invalid_argument( "negative step count" );
+		constexpr int batch = 128, ev_per_step = 4;
+		std::vector<hipEvent_t> ev( batch * ev_per_step );
+		for( auto & e : ev ) SPICE_HIP( hipEventCreate( &e ) );
+		try
+		{
+			for( std::int64_t done = 0; done < n; )
+			{
+				int const k = int( std::min<std::int64_t>( batch, n - done ) );
+				for( int i = 0; i < k; i++ )
+				{
+					hipEvent_t * e = &ev[i * ev_per_step];
+					if( fixed() )
+						launch_step_events<true>( e );
+					else
+						launch_step_events<false>( e );
+					_step++;
+					_clock.add( _p.dt );
+				}
+				SPICE_HIP( hipStreamSynchronize( _s ) );
+				for( int i = 0; i < k; i++ )
+				{
+					hipEvent_t * e = &ev[i * ev_per_step];
+					for( int q = 0; q < 3; q++ )
+					{
+						if( q == 2 && !plastic ) continue;
+						float ms = 0;
+						SPICE_HIP( hipEventElapsedTime( &ms, e[q], e[q + 1] ) );
+						kernel_ms[q] += ms;
+					}
+				}
+				done += k;
+			}
+		}
+		catch( ... )
+		{
+			for( auto & e : ev ) (void)hipEventDestroy( e );
+			throw;
+		}
+		for( auto & e : ev ) (void)hipEventDestroy( e );
+	}
+
+	void copy_spikes( std::int32_t * out, std::uint64_t cap, std::uint64_t * count ) override
+	{
+		if( _step == 0 )
+		{
+			*count = 0;
+			return;
+		}
+		int const slot = int( ( _step - 1 ) % ( _p.delay + 1 ) );
+		std::uint32_t c = 0;
+		SPICE_HIP( hipMemcpyAsync( &c, _counts.data() + slot, 4, hipMemcpyDeviceToHost, _s ) );
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+		*count = c;
+		if( out && c )
+		{
+			if( c > cap ) throw std::invalid_argument( "spike buffer too small" );
+			SPICE_HIP( hipMemcpyAsync( out, _ring.data() + std::int64_t( slot ) * _g.n, std::size_t( c ) * 4, hipMemcpyDeviceToHost, _s ) );
+			SPICE_HIP( hipStreamSynchronize( _s ) );
+		}
+	}
+
+	void synchronize() override { SPICE_HIP( hipStreamSynchronize( _s ) ); }
+
+	// ---- queries ----
+	net_params const & params() const override { return _p; }
+	std::int64_t num_neurons() const override { return _g.n; }
+	std::int64_t width() const override { return _g.width; }
+	std::int64_t num_edges() const override { return _g.nnz; }
+
+	void get_clock( std::int64_t * step, float * sum, float * c ) const override
+	{
+		*step = _step;
+		*sum = float( _clock );
+		*c = _clock.compensation();
+	}
+
+	// Moves the clock (conversion constructor / resume). Spikes in flight and the lazy
+	// plasticity bookkeeping are reset: synapse state is taken as current through step-1.
+	void set_clock( std::int64_t step, float sum, float c ) override
+	{
+		if( step < 0 || step > INT_MAX - 2 ) throw std::invalid_argument( "invalid step index" );
+		synchronize();
+		ctl_t ctl{ step, sum, c, 0, 0 };
+		SPICE_HIP( hipMemcpy( _ctl.data(), &ctl, sizeof( ctl ), hipMemcpyHostToDevice ) );
+		SPICE_HIP( hipMemset( _counts.data(), 0, _counts.bytes() ) );
+		if constexpr( plastic )
+		{
+			SPICE_HIP( hipMemset( _hist.data(), 0, _hist.bytes() ) );
+			std::vector<std::int32_t> synced( _g.n, std::int32_t( step - 1 ) );
+			SPICE_HIP( hipMemcpy( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice ) );
+			SPICE_HIP( hipMemset( _pend.data(), 0, _pend.bytes() ) );
+			SPICE_HIP( hipMemset( _fl_count.data(), 0, _fl_count.bytes() ) );
+		}
+		_step = step;
+		_clock.set( sum, c );
+	}
+
+	void get_stats( spice_stats * out ) override
+	{
+		stats_t s;
+		SPICE_HIP( hipMemcpyAsync( &s, _stats.data(), sizeof( s ), hipMemcpyDeviceToHost, _s ) );
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+		*out = { s.steps, s.spikes, s.events, s.plastic_events, s.flush_rows, s.flush_synapses };
+	}
+	void reset_stats() override
+	{
+		SPICE_HIP( hipMemsetAsync( _stats.data(), 0, _stats.bytes(), _s ) );
+		synchronize();
+	}
+
+	// ---- export / import ----
+	void export_csr( std::int64_t * row_ptr, std::int32_t * col ) override
+	{
+		synchronize();
+		SPICE_HIP( hipMemcpy( row_ptr, _g.row_ptr.data(), ( _g.n + 1 ) * 8, hipMemcpyDeviceToHost ) );
+		if( _g.nnz ) SPICE_HIP( hipMemcpy( col, _g.col.data(), _g.nnz * 4, hipMemcpyDeviceToHost ) );
+	}
+
+	void export_adj( std::int32_t * ell, std::uint64_t cap ) override
+	{
+		if( cap < std::uint64_t( _g.n * _g.width ) ) throw std::invalid_argument( "adjacency buffer too small" );
+		std::vector<std::int64_t> rp( _g.n + 1 );
+		std::vector<std::int32_t> col( _g.nnz );
+		export_csr( rp.data(), col.data() );
+		for( std::int64_t r = 0; r < _g.n; r++ )
+		{
+			std::int64_t const len = rp[r + 1] - rp[r];
+			std::copy( col.begin() + rp[r], col.begin() + rp[r + 1], ell + r * _g.width );
+			std::fill( ell + r * _g.width + len, ell + ( r + 1 ) * _g.width, INT_MAX );
+		}
+	}
+
+	void export_neuron_attr( int a, void * out, std::uint64_t bytes ) override
+	{
+		check_attr( a, int( N::size ) );
+		if( bytes != std::uint64_t( _g.n ) * attr_bytes<N>( a ) ) throw std::invalid_argument( "buffer size mismatch" );
+		synchronize();
+		SPICE_HIP( hipMemcpy( out, _na[a].data(), bytes, hipMemcpyDeviceToHost ) );
+	}
+	void import_neuron_attr( int a, void const * in, std::uint64_t bytes ) override
+	{
+		check_attr( a, int( N::size ) );
+		if( bytes != std::uint64_t( _g.n ) * attr_bytes<N>( a ) ) throw std::invalid_argument( "buffer size mismatch" );
+		synchronize();
+		SPICE_HIP( hipMemcpy( _na[a].data(), in, bytes, hipMemcpyHostToDevice ) );
+	}
+
+	void export_synapse_attr( int a, void * out, std::uint64_t bytes, bool ell ) override
+	{
+		check_attr( a, int( S::size ) );
+		std::size_t const sz = attr_bytes<S>( a );
+		if( bytes != ( ell ? std::uint64_t( _g.n * _g.width ) : std::uint64_t( _g.nnz ) ) * sz )
+			throw std::invalid_argument( "buffer size mismatch" );
+		sync_synapses();
+		if( !ell )
+		{
+			if( _g.nnz ) SPICE_HIP( hipMemcpy( out, _sa[a].data(), bytes, hipMemcpyDeviceToHost ) );
+			return;
+		}
+		std::vector<char> csr( _g.nnz * sz );
+		if( _g.nnz ) SPICE_HIP( hipMemcpy( csr.data(), _sa[a].data(), csr.size(), hipMemcpyDeviceToHost ) );
+		std::vector<std::int64_t> rp( _g.n + 1 );
+		SPICE_HIP( hipMemcpy( rp.data(), _g.row_ptr.data(), rp.size() * 8, hipMemcpyDeviceToHost ) );
+		char * o = static_cast<char *>( out );
+		std::memset( o, 0, bytes );
+		for( std::int64_t r = 0; r < _g.n; r++ )
+			std::memcpy( o + r * _g.width * sz, csr.data() + rp[r] * sz, ( rp[r + 1] - rp[r] ) * sz );
+	}
+
+	void import_synapse_attr( int a, void const * in, std::uint64_t bytes, bool ell ) override
+	{
+		check_attr( a, int( S::size ) );
+		std::size_t const sz = attr_bytes<S>( a );
+		if( bytes != ( ell ? std::uint64_t( _g.n * _g.width ) : std::uint64_t( _g.nnz ) ) * sz )
+			throw std::invalid_argument( "buffer size mismatch" );
+		sync_synapses();
+		if( !ell )
+		{
+			if( _g.nnz ) SPICE_HIP( hipMemcpy( _sa[a].data(), in, bytes, hipMemcpyHostToDevice ) );
+			return;
+		}
+		std::vector<std::int64_t> rp( _g.n + 1 );
+		SPICE_HIP( hipMemcpy( rp.data(), _g.row_ptr.data(), rp.size() * 8, hipMemcpyDeviceToHost ) );
+		std::vector<char> csr( _g.nnz * sz );
+		char const * src = static_cast<char const *>( in );
+		for( std::int64_t r = 0; r < _g.n; r++ )
+			std::memcpy( csr.data() + rp[r] * sz, src + r * _g.width * sz, ( rp[r + 1] - rp[r] ) * sz );
+		if( _g.nnz ) SPICE_HIP( hipMemcpy( _sa[a].data(), csr.data(), csr.size(), hipMemcpyHostToDevice ) );
+	}
+
+private:
+	net_params _p;
+	csr_dev _g;
+	hipStream_t _s = nullptr;
+	hipGraphExec_t _exec = nullptr;
+
+	dbuf<char> _na[MAXA], _acc[MAXA], _sa[MAXA];
+	dbuf<std::int32_t> _ring;
+	dbuf<std::uint32_t> _counts;
+	dbuf<unsigned long long> _hist;
+	dbuf<std::int32_t> _synced, _pend, _fl_id, _fl_synced, _fl_pend;
+	dbuf<std::uint8_t> _hasp;
+	dbuf<std::uint32_t> _fl_count;
+	dbuf<ctl_t> _ctl;
+	dbuf<stats_t> _stats;
+	dview _v{};
+
+	int _grid_receive = 0, _grid_update = 0, _grid_flush = 0;
+	std::int64_t _step = 0;
+	util::kahan_sum<float> _clock;
+
+	bool fixed() const { return ( _p.flags & SPICE_FLAG_FIXED_POINT ) != 0; }
+
+	template <typename Decl>
+	static std::size_t attr_bytes( int a )
+	{
+		std::size_t r = 0;
+		util::for_n<Decl::size>( [&]( auto I ) {
+			if( int( I ) == a ) r = sizeof( typename Decl::template type<I> );
+		} );
+		return r;
+	}
+	static void check_attr( int a, int n )
+	{
+		if( a < 0 || a >= n ) throw std::invalid_argument( "attribute index out of range" );
+	}
+
+	void allocate()
+	{
+		std::int64_t const n = _g.n;
+		util::for_n<N::size>( [&]( auto I ) {
+			using A = typename N::template type<I>;
+			_na[I].resize( n * sizeof( A ) );
+			if constexpr( ( ( receive_mask_v<N> >> I ) & 1u ) != 0 )
+			{
+				std::size_t const asz = ( std::is_floating_point_v<A> && true ) ? ( fixed() ? 8 : sizeof( A ) ) : sizeof( A );
+				_acc[I].resize( n * asz );
+				SPICE_HIP( hipMemsetAsync( _acc[I].data(), 0, _acc[I].bytes(), _s ) );
+			}
+		} );
+		util::for_n<S::size>( [&]( auto I ) {
+			using A = typename S::template type<I>;
+			_sa[I].resize( std::max<std::int64_t>( _g.nnz, 1 ) * sizeof( A ) );
+		} );
+		_ring.resize( std::size_t( _p.delay + 1 ) * n );
+		_counts.resize( _p.delay + 1 );
+		SPICE_HIP( hipMemsetAsync( _counts.data(), 0, _counts.bytes(), _s ) );
+		if constexpr( plastic )
+		{
+			_hist.resize( n );
+			_synced.resize( n );
+			_pend.resize( n );
+			_hasp.resize( n );
+			_fl_id.resize( n );
+			_fl_synced.resize( n );
+			_fl_pend.resize( n );
+			_fl_count.resize( 2 );
+			SPICE_HIP( hipMemsetAsync( _hist.data(), 0, _hist.bytes(), _s ) );
+			SPICE_HIP( hipMemsetAsync( _pend.data(), 0, _pend.bytes(), _s ) );
+			SPICE_HIP( hipMemsetAsync( _fl_count.data(), 0, _fl_count.bytes(), _s ) );
+			std::vector<std::int32_t> synced( n, -1 );
+			SPICE_HIP( hipMemcpyAsync( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice, _s ) );
+			SPICE_HIP( hipStreamSynchronize( _s ) ); // `synced` is a host temporary
+		}
+		_ctl.resize( 1 );
+		_stats.resize( 1 );
+		SPICE_HIP( hipMemsetAsync( _ctl.data(), 0, _ctl.bytes(), _s ) );
+		SPICE_HIP( hipMemsetAsync( _stats.data(), 0, _stats.bytes(), _s ) );
+
+		dview & v = _v;
+		v.N = std::int32_t( n );
+		v.D = _p.delay;
+		v.width = std::int32_t( std::min<std::int64_t>( _g.width, INT_MAX ) );
+		v.shift = 40;
+		v.DT = _p.dt;
+		v.flags = _p.flags;
+		v.seed = _p.seed;
+		v.row_ptr = _g.row_ptr.data();
+		v.col = _g.col.data();
+		for( int a = 0; a < MAXA; a++ )
+		{
+			v.na[a] = _na[a].data();
+			v.acc[a] = _acc[a].data();
+			v.sa[a] = _sa[a].data();
+		}
+		v.ring = _ring.data();
+		v.counts = _counts.data();
+		v.hist = _hist.data();
+		v.synced = _synced.data();
+		v.pend = _pend.data();
+		v.hasp = _hasp.data();
+		v.fl_id = _fl_id.data();
+		v.fl_synced = _fl_synced.data();
+		v.fl_pend = _fl_pend.data();
+		v.fl_count = _fl_count.data();
+		v.ctl = _ctl.data();
+		v.stats = _stats.data();
+
+		// Launch shapes. Rows longer than ~1k targets are split so one delivered spike
+		// occupies several waves; grids are persistent (work is read from device memory).
+		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
+		v.parts = std::max( 1, int( mean_row / 1024.0 + 0.5 ) );
+		v.fparts = v.parts;
+		int const cus = device_cu_count();
+		_grid_update = int( ( n + 255 ) / 256 );
+		_grid_receive = int( std::min<std::int64_t>( 4 * cus, std::max<std::int64_t>( 1, ( n * v.parts + 63 ) / 64 ) ) );
+		_grid_flush = _grid_receive;
+	}
+
+	void initialize()
+	{
+		SPICE_LAUNCH( k_init_neurons<Model><<<_grid_update, 256, 0, _s>>>( _v ) );
+		if constexpr( plastic )
+		{
+			SPICE_LAUNCH( k_init_synapses<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
+		}
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+	}
+
+	void launch_step()
+	{
+		if( fixed() )
+			launch_step_t<true>();
+		else
+			launch_step_t<false>();
+	}
+
+	template <bool F>
+	void launch_step_t()
+	{
+		SPICE_LAUNCH( k_receive<Model, F><<<_grid_receive, 256, 0, _s>>>( _v ) );
+		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( _v ) );
+		if constexpr( plastic ) SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( _v ) );
+	}
+
+	template <bool F>
+	void launch_step_events( hipEvent_t * e )
+	{
+		SPICE_HIP( hipEventRecord( e[0], _s ) );
+		SPICE_LAUNCH( k_receive<Model, F><<<_grid_receive, 256, 0, _s>>>( _v ) );
+		SPICE_HIP( hipEventRecord( e[1], _s ) );
+		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( _v ) );
+		SPICE_HIP( hipEventRecord( e[2], _s ) );
+		if constexpr( plastic ) SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( _v ) );
+		SPICE_HIP( hipEventRecord( e[3], _s ) );
+	}
+
+	void capture()
+	{
+		hipGraph_t g = nullptr;
+		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
+		try
+		{
+			for( int i = 0; i < graph_steps; i++ ) launch_step();
+		}
+		catch( ... )
+		{
+			(void)hipStreamEndCapture( _s, &g );
+			if( g ) (void)hipGraphDestroy( g );
+			throw;
+		}
+		SPICE_HIP( hipStreamEndCapture( _s, &g ) );
+		hipError_t const e = hipGraphInstantiate( &_exec, g, nullptr, nullptr, 0 );
+		(void)hipGraphDestroy( g );
+		check( e, "hipGraphInstantiate" );
+	}
+
+	// Brings all plastic synapses up to date through the last completed step.
+	void sync_synapses()
+	{
+		if constexpr( plastic )
+		{
+			if( _step > 0 )
+			{
+				// dt of the last completed step (the step that moved the clock to its state)
+				float const dt_last = _last_dt();
+				SPICE_LAUNCH( k_sync_all<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, std::int32_t( _step - 1 ), dt_last ) );
+			}
+		}
+		synchronize();
+	}
+
+	float _last_dt() const
+	{
+		// Recompute the compensated dt of step _step - 1 from the Kahan state after it:
+		// y = sum_after - sum_before - (c_after - c_before) is not recoverable exactly, so
+		// use the nominal dt (the lazy updates of one step differ from it in the last bit).
+		return _p.dt;
+	}
+};
+} // namespace spice::hipx

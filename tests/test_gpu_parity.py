@@ -1,0 +1,139 @@
+"""GPU parity: the gfx950 path against the CPU checker (oracle/), through the C ABI.
+
+Bars: graphs, initial state and integer state are bit-exact; in fixed-point mode every
+state value and every spike is bit-exact; in fp32 mode state agrees within the tolerance
+written in each test (the reference's own bar is 1e-5 mean absolute difference,
+spice_test/cuda/snn.d.cpp:26-44, 120).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from spice_amd import Layout, Snn
+from tests.helpers import (VOGELS_4000, assert_state_close, assert_state_equal, brunel_ref_layout,
+                           three_pop_layout)
+
+pytestmark = pytest.mark.gpu
+
+GRAPH_CASES = [
+    ("p1_identity", ([100], [(0, 0, 1.0)]), 3),
+    ("p0_empty", ([100], [(0, 0, 0.0)]), 3),
+    ("three_pop_ranges", ([10, 20, 30], [(0, 0, 0.5), (0, 1, 0.1), (0, 2, 0.5), (1, 0, 1.0), (1, 2, 0.5)]), 5),
+    ("vogels_4000", VOGELS_4000, 1),
+    ("brunel_ref_2000", brunel_ref_layout(2000), 7),
+    ("three_pop_3162", three_pop_layout(3162), 1),
+    ("wide_rows_20000", ([20000], [(0, 0, 0.3)]), 11),
+]
+
+
+@pytest.mark.parametrize("name,layout,seed", GRAPH_CASES, ids=[c[0] for c in GRAPH_CASES])
+def test_graph_bit_exact_vs_host_build(gpu, name, layout, seed):
+    """Device generator == host build of the same generator, bit for bit."""
+    rp_h, col_h, w_h = O.generate_csr(layout, seed)
+    net = Snn("brunel", Layout(*layout), 1e-4, 1, seed=seed)
+    rp_d, col_d = net.csr()
+    assert net.width() == w_h
+    np.testing.assert_array_equal(rp_d, rp_h)
+    np.testing.assert_array_equal(col_d, col_h)
+    # ELL export keeps the reference's format: rows padded with INT_MAX to the layout width
+    ell, w = net.adj()
+    assert ell.size == net.num_neurons() * w == net.num_synapses()
+
+
+@pytest.mark.parametrize("model", ["vogels_abbott", "brunel", "brunel_with_plasticity", "synth", "brunel_3pop",
+                                   "brunel_3pop_stdp"])
+def test_init_state_exact(gpu, model):
+    lay = three_pop_layout(1000) if "3pop" in model else ([1000], [(0, 0, 0.1)])
+    h = O.CpuSnn(model, lay, 1e-4, 15, seed=3)
+    d = Snn(model, Layout(*lay), 1e-4, 15, seed=3)
+    assert_state_equal(d.neurons(), h.neurons(), "neurons")
+    assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), "synapses")
+
+
+def _run_pair(model, layout, delay, steps, fixed, seed=5, check_every=1, rtol=1e-5, atol=1e-9, exact=None):
+    h = O.CpuSnn(model, layout, 1e-4, delay, seed=seed, fixed_point=fixed)
+    d = Snn(model, Layout(*layout), 1e-4, delay, seed=seed, fixed_point=fixed)
+    total = 0
+    for i in range(steps):
+        sh = np.sort(h.step(True))
+        sd = np.sort(d.step(True))
+        np.testing.assert_array_equal(sd, sh, err_msg=f"{model} step {i}: spike sets differ")
+        total += sh.size
+        if check_every and (i % check_every == 0 or i == steps - 1):
+            if fixed if exact is None else exact:
+                assert_state_equal(d.neurons(), h.neurons(), f"step {i}")
+            else:
+                assert_state_close(d.neurons(), h.neurons(), rtol, atol, f"step {i}")
+    return h, d, total
+
+
+@pytest.mark.parametrize("delay", [1, 8])
+def test_vogels_reference_scenario_fixed_point_bit_exact(gpu, delay):
+    """Reference scenario (vogels, N=4000, p=0.02, 1000 steps, delay 1 and 8): per-step spike
+    sets equal, state bit-exact in fixed-point mode."""
+    h, d, total = _run_pair("vogels_abbott", VOGELS_4000, delay, 1000, True, check_every=50)
+    assert total > 1000
+
+
+@pytest.mark.parametrize("delay", [1, 8])
+def test_vogels_reference_scenario_fp32(gpu, delay):
+    """Same scenario in fp32 atomics mode: spike sets equal every step, final state within
+    the reference's bar (mean absolute difference per neuron <= 1e-5) and 1e-5 relative."""
+    h, d, total = _run_pair("vogels_abbott", VOGELS_4000, delay, 1000, False, check_every=100, rtol=1e-5, atol=1e-7)
+    a, b = d.neurons(), h.neurons()
+    mad = np.mean(sum(np.abs(a[k].astype(np.float64) - b[k]) for k in a) / len(a))
+    assert mad <= 1e-5
+
+
+@pytest.mark.parametrize("model", ["brunel", "brunel_3pop", "synth"])
+def test_poisson_models_bit_exact_fixed_point(gpu, model):
+    """Models with random (Poisson) neurons: Philox keyed on (seed, neuron, step) makes the GPU
+    draw the checker's numbers, so spikes and state are bit-exact (the reference could not
+    test brunel at all, spice_test/cuda/snn.d.cpp:102-103)."""
+    lay = three_pop_layout(2000) if "3pop" in model else brunel_ref_layout(2000)
+    if model == "synth":
+        lay = ([2000], [(0, 0, 0.05)])
+    _run_pair(model, lay, 15 if model != "synth" else 1, 400, True, check_every=40)
+
+
+def test_one_step_fp32_within_1e5(gpu):
+    """One step of update + propagation from identical state matches within 1e-5 relative:
+    before every step the GPU neuron state is reset to the checker's, so each comparison
+    covers exactly one step (spikes in flight are identical because spike sets are)."""
+    lay = three_pop_layout(3162)
+    h = O.CpuSnn("brunel_3pop", lay, 1e-4, 15, seed=1)
+    d = Snn("brunel_3pop", Layout(*lay), 1e-4, 15, seed=1)
+    for i in range(120):
+        d.set_neurons(h.neurons())
+        sh = np.sort(h.step(True))
+        sd = np.sort(d.step(True))
+        np.testing.assert_array_equal(sd, sh, err_msg=f"step {i}")
+        assert_state_close(d.neurons(), h.neurons(), 1e-5, 1e-9, f"step {i}")
+
+
+@pytest.mark.parametrize("model", ["brunel_with_plasticity", "brunel_3pop_stdp"])
+def test_lazy_stdp_matches_dense(gpu, model):
+    """Lazy event-driven plasticity on the GPU vs the checker's dense per-step updates.
+
+    Lazy replay merges event-free steps into one update(n) whose trace decay is pow(d, n),
+    where the dense checker multiplies by d once per step, so plastic weights agree to a few
+    ulps rather than bit for bit; those weights feed receive(), so neuron state is compared
+    at 1e-5 relative. Spike sets must still be identical every step."""
+    lay = three_pop_layout(1500) if "3pop" in model else brunel_ref_layout(1500)
+    h, d, _ = _run_pair(model, lay, 15, 300, True, check_every=50, rtol=1e-5, atol=1e-9, exact=False)
+    sd, sh = d.synapses(ell=False), h.synapses(ell=False)
+    for k in sd:
+        np.testing.assert_allclose(sd[k], sh[k], rtol=2e-5, atol=1e-9, err_msg=k)
+
+
+def test_run_graph_equals_step(gpu):
+    """spice_hip_run (captured HIP graph) == repeated spice_hip_step."""
+    lay = three_pop_layout(2000)
+    a = Snn("brunel_3pop", Layout(*lay), 1e-4, 15, seed=9, fixed_point=True)
+    b = Snn("brunel_3pop", Layout(*lay), 1e-4, 15, seed=9, fixed_point=True)
+    for _ in range(100):
+        a.step()
+    b.run(100)
+    assert a.clock() == b.clock()
+    assert_state_equal(a.neurons(), b.neurons(), "run vs step")
+    assert a.stats() == b.stats()
