@@ -2,8 +2,13 @@
 // Three-population Brunel network: a 20 Hz Poisson input population (the first N/5
 // neurons), then excitatory (4/5 of the rest) and inhibitory LIF populations. LIF: fp32 V,
 // tau 20 ms, threshold 20 mV, reset to 10 mV, 20 refractory steps (2 ms at dt = 0.1 ms).
-// Fixed fp32 weights with the reference's 2e4 / N scaling (+2/N V from Poisson and
-// excitatory sources, -10/N V from inhibitory ones), applied to non-refractory targets.
+// Fixed fp32 weights scaling as 1/N like the reference's (+5/N V from Poisson and excitatory
+// sources, -25/N V from inhibitory ones: the reference's 1 : -5 ratio), applied to
+// non-refractory targets. The constant is 5e4 / N rather than the reference's 2e4 / N: with
+// a 20 % Poisson population at p = 0.1 the latter gives a mean external drive of
+// 0.02 N * 0.1 * 20 Hz * (2/N V) * 20 ms = 16 mV, below the 20 mV threshold, and the LIF
+// populations stay silent at every N; 5e4 / N restores the 40 mV mean external drive of the
+// reference's own Brunel topology (half Poisson, samples/main.cpp:1392-1396).
 // Intended topology: all population pairs connected at p = 0.1, delay 15 steps.
 #include <spice/models/model.h>
 
@@ -70,8 +75,8 @@ struct brunel_3pop : model
 			using util::get;
 			if( get<Twait>( dst ) <= 0 )
 			{
-				float const Wex = 0.0001f * 20'000 / info.num_neurons;  // V
-				float const Win = -0.0005f * 20'000 / info.num_neurons; // V
+				float const Wex = 0.0001f * 50'000 / info.num_neurons;  // V
+				float const Win = -0.0005f * 50'000 / info.num_neurons; // V
 				bak.atomic_add( get<V>( dst ), src < num_exc_end( info ) ? Wex : Win );
 			}
 		}

@@ -5,7 +5,7 @@
 // (1 - dt / 20 ms) and jump by 1 on a pre (spike arrival) / post (target spike) event; then
 //   W <- clamp(W + post * A+ wmax Zpre - pre * A- wmax Zpost, 0, wmax),
 // A+ = 0.01, A- = 0.0105, wmax = 3e-4 (in the 1e-4 = "standard excitatory weight" units of
-// the reference's plastic model; receive() adds W * 2e4 / N volts). Traces start at 0;
+// the reference's plastic model; receive() adds W * 5e4 / N volts, see brunel_3pop.h). Traces start at 0;
 // W starts at 1e-4 for Poisson/excitatory sources and -5e-4 for inhibitory ones (those stay
 // fixed: only E -> E synapses are plastic).
 #include <spice/models/brunel_3pop.h>
@@ -27,7 +27,7 @@ struct brunel_3pop_stdp : model
 		HYBRID static void receive( int_, Iter dst, SynIter && syn, snn_info info, Backend & bak )
 		{
 			using util::get;
-			if( get<Twait>( dst ) <= 0 ) bak.atomic_add( get<V>( dst ), get<W>( syn ) * 20'000 / info.num_neurons );
+			if( get<Twait>( dst ) <= 0 ) bak.atomic_add( get<V>( dst ), get<W>( syn ) * 50'000 / info.num_neurons );
 		}
 	};
 

@@ -33,6 +33,44 @@
 
 namespace spice::oracle
 {
+// Sequential xoshiro128+ stream seeded through the SplitMix64 finalizer: the random source of
+// the reference's CPU backend (spice/cpu/backend.h:23,52; spice/util/random.h:18-73), written
+// from the published algorithms (D. Blackman, S. Vigna, "Scrambled linear pseudorandom number
+// generators", 2018; SplitMix64 mixing function). Used only in the checker's reference-RNG
+// mode, which reproduces the reference cpu::snn trajectories of its Poisson models exactly,
+// so the checker can be pinned against the reference build.
+struct xoshiro128p
+{
+	std::uint32_t s[4];
+	static std::uint64_t mix( std::uint64_t x )
+	{
+		x = ( x ^ ( x >> 30 ) ) * 0xbf58476d1ce4e5b9ull;
+		x = ( x ^ ( x >> 27 ) ) * 0x94d049bb133111ebull;
+		return x ^ ( x >> 31 );
+	}
+	explicit xoshiro128p( std::uint64_t seed = 1 )
+	{
+		std::uint64_t h = mix( seed );
+		s[0] = std::uint32_t( h );
+		s[1] = std::uint32_t( h >> 32 );
+		h = mix( h );
+		s[2] = std::uint32_t( h );
+		s[3] = std::uint32_t( h >> 32 );
+	}
+	std::uint32_t operator()()
+	{
+		std::uint32_t const r = s[0] + s[3];
+		std::uint32_t const t = s[1] << 9;
+		s[2] ^= s[0];
+		s[3] ^= s[1];
+		s[1] ^= s[2];
+		s[0] ^= s[3];
+		s[2] ^= t;
+		s[3] = ( s[3] << 11 ) | ( s[3] >> 21 );
+		return r;
+	}
+};
+
 // Receive-target proxy for fixed-point mode: reads see the state, atomic_add accumulates.
 template <typename T, int I>
 struct fix_proxy
@@ -46,8 +84,9 @@ struct cpu_backend
 {
 	util::philox_stream rng;
 	int shift = 40;
+	xoshiro128p * seq = nullptr; // reference-RNG mode: one sequential stream per net
 
-	float rand() { return rng.rand(); }
+	float rand() { return seq ? util::u01_left( ( *seq )() ) : rng.rand(); }
 
 	template <typename T>
 	static void atomic_add( T & var, typename util::identity<T>::type val )
@@ -153,7 +192,8 @@ public:
 	    int delay,
 	    std::uint64_t seed,
 	    bool fixed,
-	    int shift )
+	    int shift,
+	    bool ref_rng = false )
 	    : _row_ptr( std::move( row_ptr ) )
 	    , _col( std::move( col ) )
 	    , _n( std::int64_t( _row_ptr.size() ) - 1 )
@@ -163,6 +203,8 @@ public:
 	    , _seed( seed )
 	    , _fixed( fixed )
 	    , _shift( shift )
+	    , _ref_rng( ref_rng )
+	    , _seq( seed )
 	{
 		if( !( dt > 0.0f ) ) throw std::invalid_argument( "dt must be positive" );
 		if( delay < 1 ) throw std::invalid_argument( "delay must be >= 1" );
@@ -170,7 +212,7 @@ public:
 		_neurons.resize( _n );
 		for( std::int64_t i = 0; i < _n; i++ )
 		{
-			cpu_backend bak{ { _seed, std::uint32_t( i ), 0u, 0u, util::TAG_NEURON_INIT }, _shift };
+			cpu_backend bak{ { _seed, std::uint32_t( i ), 0u, 0u, util::TAG_NEURON_INIT }, _shift, seq() };
 			N::init( elem_iter<ntuple>{ _neurons.data(), i }, _info, bak );
 		}
 		if constexpr( S::size > 0 )
@@ -181,7 +223,8 @@ public:
 				{
 					cpu_backend bak{
 					    { _seed, std::uint32_t( k ), std::uint32_t( std::uint64_t( k ) >> 32 ), 0u, util::TAG_SYNAPSE_INIT },
-					    _shift };
+					    _shift,
+					    seq() };
 					S::init( elem_iter<stuple>{ _synapses.data(), k }, int_( s ), _col[k], _info, bak );
 				}
 			_flags.assign( delay + 1, std::vector<char>( _n, 0 ) );
@@ -205,7 +248,7 @@ public:
 				for( std::int64_t k = _row_ptr[s]; k < _row_ptr[s + 1]; k++ )
 				{
 					int_ const dst = _col[k];
-					cpu_backend bak{ { _seed, std::uint32_t( s ), std::uint32_t( dst ), std::uint32_t( i ), util::TAG_RECEIVE }, _shift };
+					cpu_backend bak{ { _seed, std::uint32_t( s ), std::uint32_t( dst ), std::uint32_t( i ), util::TAG_RECEIVE }, _shift, seq() };
 					elem_iter<stuple const> syn{ _synapses.data(), k };
 					if( _fixed )
 						N::receive( s, fix_iter<ntuple>{ _neurons.data(), _acc, dst }, syn, _info, bak );
@@ -224,7 +267,8 @@ public:
 			if( _fixed ) apply_fixed( n );
 			cpu_backend bak{
 			    { _seed, std::uint32_t( n ), std::uint32_t( i ), std::uint32_t( std::uint64_t( i ) >> 32 ), util::TAG_NEURON_UPDATE },
-			    _shift };
+			    _shift,
+			    seq() };
 			bool const spiked = N::update( elem_iter<ntuple>{ _neurons.data(), n }, dt, _info, bak );
 			if constexpr( S::size > 0 ) _flags[i % ( D + 1 )][n] = spiked;
 			if( spiked ) _ids.push_back( int( n ) );
@@ -244,7 +288,8 @@ public:
 					if( !S::plastic( int_( s ), dst, _info ) ) continue;
 					cpu_backend bak{
 					    { _seed, std::uint32_t( k ), std::uint32_t( std::uint64_t( k ) >> 32 ), std::uint32_t( i ), util::TAG_SYNAPSE_UPDATE },
-					    _shift };
+					    _shift,
+					    seq() };
 					S::update( elem_iter<stuple>{ _synapses.data(), k }, 1, pre[s] != 0, post[dst] != 0, dt, _info, bak );
 				}
 		}
@@ -285,7 +330,11 @@ private:
 	std::uint64_t _seed;
 	bool _fixed;
 	int _shift;
+	bool _ref_rng;
+	xoshiro128p _seq;
 	snn_info _info;
+
+	xoshiro128p * seq() { return _ref_rng ? &_seq : nullptr; }
 
 	std::vector<ntuple> _neurons;
 	std::vector<stuple> _synapses;

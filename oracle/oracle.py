@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _ROOT = os.path.dirname(_HERE)
 ORACLE_LIB = os.path.join(_HERE, "liboracle.so")
 REF_LIB = os.path.join(_HERE, "_ref", "libspice_ref.so")
+REF_IEEE_LIB = os.path.join(_HERE, "_ref", "libspice_ref_ieee.so")
 
 _c = ctypes
 _P = _c.c_void_p
@@ -41,7 +42,7 @@ SYNAPSE_ATTRS = {
 }
 
 _lib = None
-_ref = None
+_refs = {}
 
 
 def build():
@@ -58,18 +59,20 @@ def lib():
     return _lib
 
 
-def ref_available():
-    return os.path.exists(REF_LIB)
+def ref_available(ieee=False):
+    return os.path.exists(REF_IEEE_LIB if ieee else REF_LIB)
 
 
-def ref():
-    global _ref
-    if _ref is None:
-        _ref = _c.CDLL(REF_LIB)
-        _ref.ref_last_error.restype = _c.c_char_p
-        _ref.ref_num_neurons.restype = _c.c_uint64
-        _ref.ref_width.restype = _c.c_uint64
-    return _ref
+def ref(ieee=False):
+    """The reference's CPU build: its own flags (-ffast-math) or the IEEE variant."""
+    path = REF_IEEE_LIB if ieee else REF_LIB
+    if path not in _refs:
+        L = _c.CDLL(path)
+        L.ref_last_error.restype = _c.c_char_p
+        L.ref_num_neurons.restype = _c.c_uint64
+        L.ref_width.restype = _c.c_uint64
+        _refs[path] = L
+    return _refs[path]
 
 
 def _check(st, L=None):
@@ -175,10 +178,23 @@ def degrees(layout, seed):
     return out
 
 
-class CpuSnn:
-    """The CPU checker: cpu::snn semantics (dense plasticity, sequential fp32 receive), Philox RNG."""
+FLAG_FIXED_POINT = 1
+FLAG_REF_RNG = 1 << 16
 
-    def __init__(self, model, layout=None, dt=1e-4, delay=1, seed=1337, fixed_point=False, shift=40, _handle=None):
+
+def _flags(fixed_point, ref_rng):
+    return (FLAG_FIXED_POINT if fixed_point else 0) | (FLAG_REF_RNG if ref_rng else 0)
+
+
+class CpuSnn:
+    """The CPU checker: cpu::snn semantics (dense plasticity, sequential fp32 receive).
+
+    Random numbers come from Philox keyed on (seed, neuron, step) like the GPU path, or, with
+    ref_rng=True, from the reference CPU backend's sequential xoshiro128+ stream seeded with
+    `seed` (to reproduce the reference build's trajectories)."""
+
+    def __init__(self, model, layout=None, dt=1e-4, delay=1, seed=1337, fixed_point=False, shift=40, _handle=None,
+                 ref_rng=False):
         self.model = model
         self._mid = MODEL_IDS[model]
         self._dt, self._delay = float(dt), int(delay)
@@ -189,11 +205,11 @@ class CpuSnn:
             pops, conns = _normalize_layout(layout)
             pa, ca = _layout_args(pops, conns)
             _check(lib().oracle_create(self._mid, _ptr(pa, _c.c_uint64), len(pops), ca, len(conns), _c.c_float(dt),
-                                       int(delay), _c.c_uint64(seed), _c.c_uint32(1 if fixed_point else 0), int(shift),
-                                       _c.byref(self._h)))
+                                       int(delay), _c.c_uint64(seed), _c.c_uint32(_flags(fixed_point, ref_rng)),
+                                       int(shift), _c.byref(self._h)))
 
     @classmethod
-    def from_csr(cls, model, row_ptr, col, width, dt, delay=1, seed=1337, fixed_point=False, shift=40):
+    def from_csr(cls, model, row_ptr, col, width, dt, delay=1, seed=1337, fixed_point=False, shift=40, ref_rng=False):
         rp = np.ascontiguousarray(row_ptr, np.int64)
         cl = np.ascontiguousarray(col, np.int32)
         if cl.size == 0:
@@ -201,17 +217,17 @@ class CpuSnn:
         h = _c.c_void_p()
         _check(lib().oracle_create_csr(MODEL_IDS[model], _c.c_void_p(rp.ctypes.data), _c.c_void_p(cl.ctypes.data),
                                        _c.c_int64(rp.size - 1), _c.c_int64(width), _c.c_float(dt), int(delay),
-                                       _c.c_uint64(seed), _c.c_uint32(1 if fixed_point else 0), int(shift),
+                                       _c.c_uint64(seed), _c.c_uint32(_flags(fixed_point, ref_rng)), int(shift),
                                        _c.byref(h)))
         return cls(model, None, dt, delay, seed, fixed_point, shift, _handle=h)
 
     @classmethod
-    def from_ell(cls, model, ell, width, dt, delay=1, seed=1337, fixed_point=False, shift=40):
+    def from_ell(cls, model, ell, width, dt, delay=1, seed=1337, fixed_point=False, shift=40, ref_rng=False):
         e = np.ascontiguousarray(ell, np.int32)
         h = _c.c_void_p()
         _check(lib().oracle_create_ell(MODEL_IDS[model], _c.c_void_p(e.ctypes.data), _c.c_int64(e.size // width),
                                        _c.c_int64(width), _c.c_float(dt), int(delay), _c.c_uint64(seed),
-                                       _c.c_uint32(1 if fixed_point else 0), int(shift), _c.byref(h)))
+                                       _c.c_uint32(_flags(fixed_point, ref_rng)), int(shift), _c.byref(h)))
         return cls(model, None, dt, delay, seed, fixed_point, shift, _handle=h)
 
     def close(self):
@@ -325,9 +341,14 @@ class CpuSnn:
 
 
 class RefSnn:
-    """The reference's own cpu::snn (oracle/_ref), for the four reference models."""
+    """The reference's own cpu::snn (oracle/_ref), for the four reference models.
 
-    def __init__(self, model, layout, dt, delay=1):
+    The reference seeds the random stream of each constructed net from a process-wide counter
+    starting at 1337 (spice/cpu/snn.h.cpp:16, 76); `self.seed` is the value this net got."""
+
+    _created = {}
+
+    def __init__(self, model, layout, dt, delay=1, ieee=False):
         assert model in ("vogels_abbott", "brunel", "brunel_with_plasticity", "synth")
         self.model = model
         pops, conns = _normalize_layout(layout)
@@ -336,37 +357,41 @@ class RefSnn:
         cd = np.asarray([c[1] for c in conns] or [0], np.uint64)
         cp = np.asarray([c[2] for c in conns] or [0], np.float32)
         self._h = _c.c_void_p()
-        st = ref().ref_create(MODEL_IDS[model], _c.c_void_p(pa.ctypes.data), len(pops), _c.c_void_p(cs.ctypes.data),
+        self._L = L = ref(ieee)
+        st = L.ref_create(MODEL_IDS[model], _c.c_void_p(pa.ctypes.data), len(pops), _c.c_void_p(cs.ctypes.data),
                               _c.c_void_p(cd.ctypes.data), _c.c_void_p(cp.ctypes.data), len(conns), _c.c_float(dt),
                               int(delay), _c.byref(self._h))
         if st:
-            raise ValueError(ref().ref_last_error().decode())
+            raise ValueError(L.ref_last_error().decode())
+        k = RefSnn._created.get(ieee, 0)
+        self.seed = 1337 + k
+        RefSnn._created[ieee] = k + 1
         self._dt, self._delay = float(dt), int(delay)
 
     def __del__(self):
         try:
             if self._h:
-                ref().ref_destroy(self._h)
+                self._L.ref_destroy(self._h)
         except Exception:
             pass
 
     def num_neurons(self):
-        return ref().ref_num_neurons(self._h)
+        return self._L.ref_num_neurons(self._h)
 
     def width(self):
-        return ref().ref_width(self._h)
+        return self._L.ref_width(self._h)
 
     def step(self, spikes=True):
         n = self.num_neurons()
         buf = np.empty(max(n, 1), np.int32)
         cnt = _c.c_uint64()
-        ref().ref_step(self._h, _c.c_void_p(buf.ctypes.data), _c.c_uint64(n), _c.byref(cnt))
+        self._L.ref_step(self._h, _c.c_void_p(buf.ctypes.data), _c.c_uint64(n), _c.byref(cnt))
         return buf[: cnt.value].copy()
 
     def adj(self):
         n, w = self.num_neurons(), self.width()
         e = np.empty(n * w, np.int32)
-        ref().ref_export_adj(self._h, _c.c_void_p(e.ctypes.data))
+        self._L.ref_export_adj(self._h, _c.c_void_p(e.ctypes.data))
         return e, w
 
     def neurons(self):
@@ -374,7 +399,7 @@ class RefSnn:
         out = {}
         for a, (name, dt) in enumerate(NEURON_ATTRS[self.model]):
             arr = np.empty(n, dt)
-            ref().ref_export_neurons(self._h, a, _c.c_void_p(arr.ctypes.data))
+            self._L.ref_export_neurons(self._h, a, _c.c_void_p(arr.ctypes.data))
             out[name] = arr
         return out
 
@@ -383,7 +408,7 @@ class RefSnn:
         out = {}
         for a, (name, dt) in enumerate(SYNAPSE_ATTRS.get(self.model, [])):
             arr = np.empty(n * w, dt)
-            ref().ref_export_synapses(self._h, a, _c.c_void_p(arr.ctypes.data))
+            self._L.ref_export_synapses(self._h, a, _c.c_void_p(arr.ctypes.data))
             out[name] = arr
         return out
 

@@ -21,6 +21,10 @@ using namespace spice;
 
 static thread_local std::string g_err;
 
+// Checker-only flag: draw random numbers from the reference CPU backend's sequential
+// xoshiro128+ stream seeded with `seed` (the reference seeds nets 1337, 1338, ...).
+constexpr uint32_t ORACLE_FLAG_REF_RNG = 1u << 16;
+
 template <typename F>
 static int guard( F && f )
 {
@@ -65,7 +69,15 @@ static oracle::net_base * make_net(
 	return dispatch_model( model, [&]( auto m ) -> oracle::net_base * {
 		using M = decltype( m );
 		return new oracle::cpu_snn<M>(
-		    std::move( rp ), std::move( col ), width, dt, delay, seed, ( flags & SPICE_FLAG_FIXED_POINT ) != 0, shift );
+		    std::move( rp ),
+		    std::move( col ),
+		    width,
+		    dt,
+		    delay,
+		    seed,
+		    ( flags & SPICE_FLAG_FIXED_POINT ) != 0,
+		    shift,
+		    ( flags & ORACLE_FLAG_REF_RNG ) != 0 );
 	} );
 }
 
