@@ -47,24 +47,32 @@ WORKLOADS = {
 
 def algorithmic_bytes(model, n, st, steps):
     """Algorithmic HBM bytes per launch of each kernel, from the device counters of the
-    profiled steps. Per delivered spike: its id (4) + row offsets (16); per delivered event:
-    target id (4) + synapse weight read (4, models with per-synapse state); per plastic
-    synapse brought up to date: read + write of its 12-byte state minus the weight read
-    already counted (20). Neuron update: state read + write, accumulator read + clear, and for
-    plastic models history (16) and row bookkeeping (9) per neuron. Flush: target ids of the
-    flushed rows' plastic synapses (4) + their 12-byte state read + write (24). Target-neuron
-    accumulation is counted as atomics, not bytes."""
+    profiled steps (per-neuron arrays that stay L2-resident -- spike history, traces' current
+    values -- are counted once per neuron in the update kernel, not per synapse):
+      receive: per delivered spike its id (4) and row offsets (16); per delivered event the
+               target id (4) and, for models with per-synapse state, the weight read (4); per
+               plastic synapse changed, the write-back of its state (4 for neuron-trace
+               synapses, 12 + 8 re-read for per-synapse-trace synapses);
+      update : neuron state read + write, accumulator read + clear, plus for plastic models
+               history (16), row bookkeeping (9) and, for neuron-trace synapses, traces (16)
+               and trace rings (8) per neuron; 4 per emitted spike;
+      flush  : target ids of every scanned slot of the flushed rows (4) + per changed synapse
+               its state read + write (8 neuron-trace, 24 per-synapse-trace).
+    Target-neuron accumulation is counted as atomics, not bytes."""
     from spice_amd.models import MODELS
     _, nattrs, sattrs = MODELS[model]
     s_n = sum(np.dtype(t).itemsize for _, t in nattrs)
+    s_s = sum(np.dtype(t).itemsize for _, t in sattrs)
     plastic = bool(sattrs)
+    traced = model == "brunel_3pop_stdp"
     acc_bytes = 4 * (1 if "brunel" in model else 2)  # receive-target attributes per neuron
     spk = st["spikes"] / steps
     ev = st["events"] / steps
     pev = st["plastic_events"] / steps
-    recv = spk * 20 + ev * (8 if plastic else 4) + pev * (20 if plastic else 0)
-    upd = n * (2 * s_n + 2 * acc_bytes + (25 if plastic else 0)) + spk * 4
-    fl = (st["flush_synapses"] / steps) * 28 if plastic else 0.0
+    wb = 4 if traced else (2 * s_s - 4)
+    recv = spk * 20 + ev * (8 if plastic else 4) + pev * (wb if plastic else 0)
+    upd = n * (2 * s_n + 2 * acc_bytes + (25 if plastic else 0) + (24 if traced else 0)) + spk * 4
+    fl = (st["flush_slots"] * 4 + st["flush_synapses"] * (2 * s_s)) / steps if plastic else 0.0
     return {"receive": recv, "update": upd, "flush": fl}
 
 

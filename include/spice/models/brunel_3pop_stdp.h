@@ -1,13 +1,13 @@
 #pragma once
 // brunel_3pop with additive pair-based STDP on excitatory -> excitatory synapses.
 //
-// Each synapse carries its weight W and pre/post traces. Over each step the traces decay by
-// (1 - dt / 20 ms) and jump by 1 on a pre (spike arrival) / post (target spike) event; then
-//   W <- clamp(W + post * A+ wmax Zpre - pre * A- wmax Zpost, 0, wmax),
+// Neuron-local pre/post traces (tau = 20 ms; decay (1 - dt/tau) per step, +1 per spike
+// arrival / emission, see trace_synapse in model.h) drive the weight of each plastic synapse:
+//   W <- clamp(W + post * A+ wmax x_pre - pre * A- wmax y_post, 0, wmax),
 // A+ = 0.01, A- = 0.0105, wmax = 3e-4 (in the 1e-4 = "standard excitatory weight" units of
-// the reference's plastic model; receive() adds W * 5e4 / N volts, see brunel_3pop.h). Traces start at 0;
+// the reference's plastic model; receive() adds W * 5e4 / N volts, see brunel_3pop.h).
 // W starts at 1e-4 for Poisson/excitatory sources and -5e-4 for inhibitory ones (those stay
-// fixed: only E -> E synapses are plastic).
+// fixed: only E -> E synapses are plastic). Per synapse only W is stored.
 #include <spice/models/brunel_3pop.h>
 
 namespace spice
@@ -16,9 +16,7 @@ struct brunel_3pop_stdp : model
 {
 	enum syn_attr
 	{
-		W,
-		Zpre,
-		Zpost
+		W
 	};
 
 	struct neuron : brunel_3pop::neuron
@@ -31,34 +29,25 @@ struct brunel_3pop_stdp : model
 		}
 	};
 
-	struct synapse : ::spice::synapse<float, float, float>
+	struct synapse : ::spice::trace_synapse<float>
 	{
 		template <typename Iter, typename Backend>
 		HYBRID static void init( Iter syn, int_ src, int_, snn_info info, Backend & )
 		{
-			using util::get;
-			get<W>( syn ) = src < brunel_3pop::num_exc_end( info ) ? 0.0001f : -0.0005f;
-			get<Zpre>( syn ) = 0.0f;
-			get<Zpost>( syn ) = 0.0f;
+			util::get<W>( syn ) = src < brunel_3pop::num_exc_end( info ) ? 0.0001f : -0.0005f;
 		}
 
+		HYBRID static float trace_decay( float dt ) { return 1.0f - dt * ( 1.0f / 0.02f ); }
+
 		template <typename Iter, typename Backend>
-		HYBRID static void update(
-		    Iter && syn, int_ const nsteps, bool const pre, bool const post, float const dt, snn_info, Backend & bak )
+		HYBRID static void on_event( Iter && syn, bool const pre, bool const post, float const x, float const y, snn_info, Backend & bak )
 		{
 			using util::get;
-			float const TstdpInv = 1.0f / 0.02f;
 			float const Aplus = 0.01f * 0.0003f;
 			float const Aminus = 0.0105f * 0.0003f;
 			float const wmax = 0.0003f;
-			float const decay = bak.pow( 1 - dt * TstdpInv, nsteps );
-
-			float const zpre = bak.fma( get<Zpre>( syn ), decay, pre );
-			float const zpost = bak.fma( get<Zpost>( syn ), decay, post );
-			get<Zpre>( syn ) = zpre;
-			get<Zpost>( syn ) = zpost;
 			float const w = get<W>( syn );
-			get<W>( syn ) = bak.clamp( w + post * Aplus * zpre - pre * Aminus * zpost, 0.0f, wmax );
+			get<W>( syn ) = bak.clamp( w + post * Aplus * x - pre * Aminus * y, 0.0f, wmax );
 		}
 
 		HYBRID static bool plastic( int_ const src, int_ const dst, snn_info const info )

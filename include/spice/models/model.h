@@ -73,6 +73,46 @@ struct synapse : util::type_list<Ts...>
 	HYBRID static bool plastic( int_, int_, snn_info ) { return true; }
 };
 
+// Synapse whose plasticity is driven by two neuron-local traces (pair-based STDP and its
+// relatives). The framework keeps, per neuron,
+//   x(t) = fma(x(t-1), trace_decay(dt), arrived(t))   pre trace: a spike of this neuron
+//                                                      reaches its synapses at step t
+//   y(t) = fma(y(t-1), trace_decay(dt), spiked(t))    post trace
+// (both start at 0), and every step, for every plastic synapse src -> dst, calls
+//   on_event(syn, pre = arrived_src(t), post = spiked_dst(t), x_src(t), y_dst(t), info, backend)
+// The synapse state holds only what the rule needs per synapse (typically the weight).
+// on_event must leave the synapse unchanged when pre and post are both false; that is what
+// lets the GPU apply it only at events (lazily) with results identical to the dense
+// per-step definition. The synapse::update of the generic contract is not used.
+template <typename... Ts>
+struct trace_synapse : synapse<Ts...>
+{
+	static constexpr bool neuron_traces = true;
+
+	HYBRID static float trace_decay( float dt ) { return 1.0f - dt * ( 1.0f / 0.02f ); }
+
+	template <typename Iter, typename Backend>
+	HYBRID static void on_event( Iter &&, bool, bool, float, float, snn_info, Backend & )
+	{
+	}
+};
+
+namespace detail
+{
+template <typename S, typename = void>
+struct has_neuron_traces : std::false_type
+{
+};
+template <typename S>
+struct has_neuron_traces<S, std::void_t<decltype( S::neuron_traces )>> : std::bool_constant<S::neuron_traces>
+{
+};
+} // namespace detail
+
+// True for synapse types derived from trace_synapse<...>.
+template <typename Synapse>
+inline constexpr bool has_neuron_traces_v = detail::has_neuron_traces<Synapse>::value;
+
 struct model
 {
 	struct neuron : ::spice::neuron<>

@@ -89,9 +89,10 @@ typedef struct
 	uint64_t steps;          /* completed steps */
 	uint64_t spikes;         /* spikes emitted */
 	uint64_t events;         /* synaptic events delivered (real edges of delivered spikes) */
-	uint64_t plastic_events; /* plastic synapses brought up to date on spike delivery */
+	uint64_t plastic_events; /* plastic synapses changed when brought up to date on delivery */
 	uint64_t flush_rows;     /* rows flushed because their post-spike history window filled */
-	uint64_t flush_synapses; /* plastic synapses touched by those flushes */
+	uint64_t flush_synapses; /* plastic synapses changed by those flushes */
+	uint64_t flush_slots;    /* synapse slots (row entries) scanned by those flushes */
 } spice_stats;
 
 /* Thread-local message of the last failed call on this thread ("" if none). */

@@ -171,6 +171,7 @@ struct net_base
 	virtual void export_synapse_attr( int a, void * out ) const = 0; // CSR order
 	virtual void import_synapse_attr( int a, void const * in ) = 0;
 	virtual void get_clock( std::int64_t * i, float * sum, float * c ) const = 0;
+	virtual bool export_traces( float * x, float * y ) const = 0;
 	virtual void set_clock( std::int64_t i, float sum, float c ) = 0;
 };
 
@@ -228,6 +229,11 @@ public:
 					S::init( elem_iter<stuple>{ _synapses.data(), k }, int_( s ), _col[k], _info, bak );
 				}
 			_flags.assign( delay + 1, std::vector<char>( _n, 0 ) );
+			if constexpr( has_neuron_traces_v<S> )
+			{
+				_x.assign( _n, 0.0f );
+				_y.assign( _n, 0.0f );
+			}
 		}
 		for( auto & a : _acc ) a.assign( _fixed ? _n : 0, 0 );
 	}
@@ -277,7 +283,30 @@ public:
 		if( out ) out->assign( _ids.begin() + before, _ids.end() );
 
 		// 3. dense plasticity
-		if constexpr( S::size > 0 )
+		if constexpr( S::size > 0 && has_neuron_traces_v<S> )
+		{
+			// neuron-local traces, then every plastic synapse with this step's flags
+			auto const & pre = _flags[( i + 1 ) % ( D + 1 )];
+			auto const & post = _flags[i % ( D + 1 )];
+			float const d = S::trace_decay( dt );
+			for( std::int64_t n = 0; n < _n; n++ )
+			{
+				_x[n] = std::fma( _x[n], d, pre[n] ? 1.0f : 0.0f );
+				_y[n] = std::fma( _y[n], d, post[n] ? 1.0f : 0.0f );
+			}
+			for( std::int64_t s = 0; s < _n; s++ )
+				for( std::int64_t k = _row_ptr[s]; k < _row_ptr[s + 1]; k++ )
+				{
+					int_ const dst = _col[k];
+					if( !S::plastic( int_( s ), dst, _info ) ) continue;
+					cpu_backend bak{
+					    { _seed, std::uint32_t( k ), std::uint32_t( std::uint64_t( k ) >> 32 ), std::uint32_t( i ), util::TAG_SYNAPSE_UPDATE },
+					    _shift,
+					    seq() };
+					S::on_event( elem_iter<stuple>{ _synapses.data(), k }, pre[s] != 0, post[dst] != 0, _x[s], _y[dst], _info, bak );
+				}
+		}
+		else if constexpr( S::size > 0 )
 		{
 			auto const & pre = _flags[( i + 1 ) % ( D + 1 )];
 			auto const & post = _flags[i % ( D + 1 )];
@@ -308,6 +337,14 @@ public:
 	void import_neuron_attr( int a, void const * in ) override { copy_in( _neurons, a, in ); }
 	void export_synapse_attr( int a, void * out ) const override { copy_out( _synapses, a, out ); }
 	void import_synapse_attr( int a, void const * in ) override { copy_in( _synapses, a, in ); }
+
+	bool export_traces( float * x, float * y ) const override
+	{
+		if( _x.empty() ) return false;
+		std::copy( _x.begin(), _x.end(), x );
+		std::copy( _y.begin(), _y.end(), y );
+		return true;
+	}
 
 	void get_clock( std::int64_t * i, float * sum, float * c ) const override
 	{
@@ -341,6 +378,7 @@ private:
 	std::vector<int> _ids;
 	std::vector<std::int64_t> _counts;
 	std::vector<std::vector<char>> _flags;
+	std::vector<float> _x, _y; // neuron-local pre/post traces (trace synapses)
 	std::vector<std::int64_t> _acc[N::size > 0 ? N::size : 1];
 
 	std::int64_t _i = 0;

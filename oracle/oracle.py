@@ -38,7 +38,7 @@ NEURON_ATTRS = {
 }
 SYNAPSE_ATTRS = {
     "brunel_with_plasticity": [("W", np.float32), ("Zpre", np.float32), ("Zpost", np.float32)],
-    "brunel_3pop_stdp": [("W", np.float32), ("Zpre", np.float32), ("Zpost", np.float32)],
+    "brunel_3pop_stdp": [("W", np.float32)],
 }
 
 _lib = None
@@ -330,6 +330,14 @@ class CpuSnn:
         for a, (name, dt) in enumerate(SYNAPSE_ATTRS.get(self.model, [])):
             arr = np.ascontiguousarray(state[name], dt)
             _check(lib().oracle_import_synapse_attr(self._h, a, _c.c_void_p(arr.ctypes.data)))
+
+    def traces(self):
+        """Neuron-local (pre, post) traces of trace-synapse models."""
+        n = self.num_neurons()
+        x, y = np.empty(n, np.float32), np.empty(n, np.float32)
+        if lib().oracle_export_traces(self._h, _c.c_void_p(x.ctypes.data), _c.c_void_p(y.ctypes.data)):
+            raise ValueError("model has no neuron-local traces")
+        return x, y
 
     def clock(self):
         i, s, c = _c.c_int64(), _c.c_float(), _c.c_float()

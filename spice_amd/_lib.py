@@ -37,7 +37,7 @@ class Edge(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
-                ("steps", "spikes", "events", "plastic_events", "flush_rows", "flush_synapses")]
+                ("steps", "spikes", "events", "plastic_events", "flush_rows", "flush_synapses", "flush_slots")]
 
 
 _P = ctypes.c_void_p

@@ -122,4 +122,28 @@ __device__ inline std::uint64_t wave_inclusive_scan( std::uint64_t x )
 	return x;
 }
 __device__ inline std::uint64_t wave_sum( std::uint64_t x ) { return shfl_u64( wave_inclusive_scan( x ), 63 ); }
+
+// Butterfly min / max over the 64 lanes (every lane gets the result).
+__device__ inline std::int64_t wave_min( std::int64_t x )
+{
+	int const l = lane_id();
+#pragma unroll
+	for( int d = 1; d < 64; d <<= 1 )
+	{
+		std::int64_t const y = std::int64_t( shfl_u64( std::uint64_t( x ), l ^ d ) );
+		x = y < x ? y : x;
+	}
+	return x;
+}
+__device__ inline std::int64_t wave_max( std::int64_t x )
+{
+	int const l = lane_id();
+#pragma unroll
+	for( int d = 1; d < 64; d <<= 1 )
+	{
+		std::int64_t const y = std::int64_t( shfl_u64( std::uint64_t( x ), l ^ d ) );
+		x = y > x ? y : x;
+	}
+	return x;
+}
 } // namespace spice::hipx

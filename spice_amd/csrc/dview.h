@@ -14,26 +14,41 @@
 //   synced   int32[N]    row s is up to date through this step
 //   pend     int32[N]    row s has an undelivered pre event at synced+1
 //   hasp     uint8[N]    row s has at least one plastic synapse
+//   pl_lo/hi int32[N]    first / one-past-last plastic slot of row s (offsets in the row)
 //   flush_*  int32[N]    rows whose history window is full this step, with their
 //                        (synced, pend) at that time; counters double-buffered by step parity
+//   pivots   int32[N * (ntiles+1)]  per row and target tile: offset of the tile's first target
+//   xtr/ytr  f32[N]      neuron-local pre/post traces (trace synapses)
+//   xh/yh    f32[64 * N] their values over the last 64 steps (ring, slot = step mod 64)
 #include <cstdint>
 
 namespace spice::hipx
 {
 constexpr int MAXA = 16;
 
+// Device clock. Two slots: the kernels of step T read slot T & 1 and the neuron-update
+// kernel writes step T + 1's clock into the other slot, so the clock advances without
+// atomics or an extra kernel, and a captured graph of an even number of steps replays.
 struct ctl_t
 {
-	std::int64_t step; // index of the step being executed / next to execute
+	std::int64_t step; // index of the step being executed
 	float sum;         // Kahan sum of dt over completed steps
 	float c;           // Kahan compensation
-	std::uint32_t done;
-	std::uint32_t pad;
 };
 
-struct stats_t
+// Device counters live in per-block slots: block b of any step kernel adds its block-reduced
+// counts to bstats[b * NSTAT + k] with a plain read-add-write (no other block touches the
+// slot, kernels of a stream never overlap), so no same-address atomics are needed. The host
+// sums the slots when it reads statistics.
+enum stat_index
 {
-	unsigned long long steps, spikes, events, plastic_events, flush_rows, flush_synapses;
+	ST_SPIKES = 0,
+	ST_EVENTS,
+	ST_PLASTIC,
+	ST_FLUSH_ROWS,
+	ST_FLUSH_SYN,
+	ST_FLUSH_SLOTS,
+	NSTAT = 8
 };
 
 struct dview
@@ -59,12 +74,22 @@ struct dview
 	std::int32_t * synced;
 	std::int32_t * pend;
 	std::uint8_t * hasp;
+	std::int32_t * pl_lo; // plastic slots of row r lie in [row_ptr[r] + pl_lo[r], row_ptr[r] + pl_hi[r])
+	std::int32_t * pl_hi;
 	std::int32_t * fl_id;
 	std::int32_t * fl_synced;
 	std::int32_t * fl_pend;
 	std::uint32_t * fl_count; // [2]
+	float * xtr;
+	float * ytr;
+	float * xh;
+	float * yh;
 
-	ctl_t * ctl;
-	stats_t * stats;
+	ctl_t * ctl;      // this step's clock
+	ctl_t * ctl_next; // the next step's clock (written by k_update)
+	unsigned long long * bstats; // [nblocks_max * NSTAT]
+
+	std::int32_t * pivots; // [N][ntiles + 1]: offset in row r of its first target >= t * tile_w
+	std::int32_t ntiles, tile_w;
 };
 } // namespace spice::hipx

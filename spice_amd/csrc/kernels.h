@@ -158,26 +158,29 @@ __device__ inline snn_info info_of( dview const & v )
 }
 __device__ inline float step_dt( dview const & v ) { return v.DT - v.ctl->c; }
 
-// Called by every thread at the end of the last kernel of a step: the last block to finish
-// advances the device clock exactly as kahan_sum::add does on the host.
-__device__ inline void finish_step( dview const & v )
+// Adds per-lane counts to this block's statistics slot: wave reduction, then the waves'
+// totals through LDS, then one plain read-add-write by thread 0. Every thread must call it.
+template <int K>
+__device__ inline void block_add_stats( dview const & v, unsigned long long const ( &x )[K], int const ( &idx )[K] )
 {
+	__shared__ unsigned long long part[4][K]; // blockDim.x <= 256: at most 4 waves
+	int const w = threadIdx.x >> 6;
+#pragma unroll
+	for( int k = 0; k < K; k++ )
+	{
+		unsigned long long const t = wave_sum( x[k] );
+		if( lane_id() == 0 ) part[w][k] = t;
+	}
 	__syncthreads();
 	if( threadIdx.x == 0 )
 	{
-		__threadfence();
-		unsigned const prev = atomicAdd( &v.ctl->done, 1u );
-		if( prev == gridDim.x - 1 )
+		int const nw = int( blockDim.x + 63 ) >> 6;
+#pragma unroll
+		for( int k = 0; k < K; k++ )
 		{
-			__threadfence();
-			ctl_t * c = v.ctl;
-			float const y = v.DT - c->c;
-			float const t = c->sum + y;
-			c->c = ( t - c->sum ) - y;
-			c->sum = t;
-			c->step = c->step + 1;
-			c->done = 0;
-			v.stats->steps += 1;
+			unsigned long long t = 0;
+			for( int i = 0; i < nw; i++ ) t += part[i][k];
+			if( t ) v.bstats[std::int64_t( blockIdx.x ) * NSTAT + idx[k]] += t;
 		}
 	}
 }
@@ -234,6 +237,49 @@ __device__ inline void replay(
 	if( t < now ) Syn::update( syn, now - t, false, false, dt, info, bak );
 }
 
+// Neuron-local-trace plasticity (trace_synapse models): same event sequence as replay(), but
+// an event calls Syn::on_event with the source's pre trace and the target's post trace at
+// the event's step, read from the 64-step trace rings. Event-free steps change nothing by
+// contract, so a synapse whose target did not spike in the window (and with no pending pre
+// event) is not touched at all. Returns whether the synapse changed.
+__device__ inline unsigned long long window_mask( std::int32_t n ) { return n >= 64 ? ~0ull : ( ( 1ull << n ) - 1ull ); }
+
+template <typename Syn, typename Iter, typename Bak>
+__device__ inline bool replay_traced(
+    Iter syn,
+    unsigned long long hist,
+    std::int32_t now,
+    std::int32_t synced,
+    std::int32_t pend,
+    std::int32_t s,
+    std::int32_t j,
+    dview const & v,
+    snn_info info,
+    Bak & bak )
+{
+	std::int32_t const n = now - synced;
+	if( n <= 0 ) return false;
+	unsigned long long h = hist & window_mask( n );
+	if( !pend && !h ) return false;
+	std::int64_t const N = v.N;
+	if( pend )
+	{
+		int const b = n - 1;
+		bool const post = ( h >> b ) & 1ull;
+		h &= ~( 1ull << b );
+		std::int64_t const slot = ( synced + 1 ) & ( HISTORY - 1 );
+		Syn::on_event( syn, true, post, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
+	}
+	while( h )
+	{
+		int const b = 63 - __clzll( h );
+		std::int64_t const slot = ( now - b ) & ( HISTORY - 1 );
+		Syn::on_event( syn, false, true, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
+		h &= ~( 1ull << b );
+	}
+	return true;
+}
+
 // ---- kernels -------------------------------------------------------------------------------
 
 template <typename Model>
@@ -257,7 +303,7 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 	if( r >= v.N ) return;
 	snn_info const info = info_of( v );
 	std::int64_t const lo = v.row_ptr[r], hi = v.row_ptr[r + 1];
-	bool anyp = false;
+	std::int64_t first = hi, last = lo - 1;
 	for( std::int64_t e = lo + lane_id(); e < hi; e += 64 )
 	{
 		std::int32_t const j = v.col[e];
@@ -266,10 +312,20 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 		    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), 0u, util::TAG_SYNAPSE_INIT );
 		S::init( reg_iter<typename S::tuple_t>{ s, std::uint32_t( e ) }, int_( r ), j, info, bak );
 		store_tuple<S>( s, v.sa, e );
-		anyp |= S::plastic( int_( r ), j, info );
+		if( S::plastic( int_( r ), j, info ) )
+		{
+			first = e < first ? e : first;
+			last = e;
+		}
 	}
-	std::uint64_t const m = ballot( anyp );
-	if( lane_id() == 0 ) v.hasp[r] = m != 0;
+	first = wave_min( first );
+	last = wave_max( last );
+	if( lane_id() == 0 )
+	{
+		v.hasp[r] = last >= first;
+		v.pl_lo[r] = last >= first ? std::int32_t( first - lo ) : 0;
+		v.pl_hi[r] = last >= first ? std::int32_t( last + 1 - lo ) : 0;
+	}
 }
 
 template <typename Model, bool FIXED>
@@ -280,10 +336,9 @@ __global__ __launch_bounds__( 256 ) void k_receive( dview v )
 	constexpr bool plastic = is_plastic_v<Model>;
 
 	std::int64_t const T = v.ctl->step;
-	if( T < v.D ) return;
 	int const D1 = v.D + 1;
 	int const slot = int( ( T + 1 ) % D1 ); // ring entry of step T - D
-	std::int64_t const nitems = std::int64_t( v.counts[slot] ) * v.parts;
+	std::int64_t const nitems = T < v.D ? 0 : std::int64_t( v.counts[slot] ) * v.parts;
 	std::int32_t const * spk = v.ring + std::int64_t( slot ) * v.N;
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
@@ -298,48 +353,312 @@ __global__ __launch_bounds__( 256 ) void k_receive( dview v )
 		std::int64_t const p = it - k * v.parts;
 		std::int32_t const s = uniform( spk[k] );
 		std::int64_t const lo = v.row_ptr[s], hi = v.row_ptr[s + 1];
-		std::int64_t const chunk = ( ( hi - lo + v.parts - 1 ) / v.parts + 63 ) & ~std::int64_t( 63 );
+		std::int64_t const chunk = ( ( hi - lo + v.parts - 1 ) / v.parts + 255 ) & ~std::int64_t( 255 );
 		std::int64_t const beg = lo + p * chunk;
 		std::int64_t const end = beg + chunk < hi ? beg + chunk : hi;
 		if( beg >= end ) continue;
 		ev += std::uint64_t( end - beg );
 
+		constexpr int U = 4; // slots per lane per iteration: independent loads in flight
+		[[maybe_unused]] std::int32_t synced = 0, pend = 0;
 		if constexpr( plastic )
 		{
-			std::int32_t const synced = v.synced[s];
-			std::int32_t const pend = v.pend[s];
-			for( std::int64_t e = beg + lane; e < end; e += 64 )
-			{
-				std::int32_t const j = v.col[e];
-				typename S::tuple_t sy;
-				load_tuple<S>( sy, v.sa, e );
-				auto bak = make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j ), std::uint32_t( T ), util::TAG_RECEIVE );
-				if( S::plastic( s, j, info ) )
-				{
-					replay<S>( reg_iter<typename S::tuple_t>{ sy, std::uint32_t( e ) }, v.hist[j], std::int32_t( T - 1 ), synced, pend, dt, info, bak );
-					store_tuple<S>( sy, v.sa, e );
-					pev++;
-				}
-				N::receive( s, recv_iter<N>{ &v, j }, reg_iter<typename S::tuple_t>{ sy, std::uint32_t( e ) }, info, bak );
-			}
+			synced = v.synced[s];
+			pend = v.pend[s];
 		}
-		else
+		for( std::int64_t base = beg; base < end; base += 64 * U )
 		{
-			for( std::int64_t e = beg + lane; e < end; e += 64 )
+			std::int32_t j[U];
+			bool ok[U];
+#pragma unroll
+			for( int u = 0; u < U; u++ )
 			{
-				std::int32_t const j = v.col[e];
-				auto bak = make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j ), std::uint32_t( T ), util::TAG_RECEIVE );
-				N::receive( s, recv_iter<N>{ &v, j }, no_syn_iter{}, info, bak );
+				std::int64_t const e = base + lane + 64 * u;
+				ok[u] = e < end;
+				j[u] = ok[u] ? v.col[e] : 0;
+			}
+			if constexpr( plastic )
+			{
+				typename S::tuple_t sy[U];
+				bool pl[U];
+				unsigned long long hj[U];
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					if( ok[u] ) load_tuple<S>( sy[u], v.sa, base + lane + 64 * u );
+					pl[u] = ok[u] && S::plastic( s, j[u], info );
+					hj[u] = pl[u] ? v.hist[j[u]] : 0ull;
+				}
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					if( !ok[u] ) continue;
+					std::int64_t const e = base + lane + 64 * u;
+					auto bak = make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE );
+					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e ) };
+					if( pl[u] )
+					{
+						bool changed;
+						if constexpr( has_neuron_traces_v<S> )
+							changed = replay_traced<S>( itr, hj[u], std::int32_t( T - 1 ), synced, pend, s, j[u], v, info, bak );
+						else
+						{
+							replay<S>( itr, hj[u], std::int32_t( T - 1 ), synced, pend, dt, info, bak );
+							changed = true;
+						}
+						if( changed )
+						{
+							store_tuple<S>( sy[u], v.sa, e );
+							pev++;
+						}
+					}
+					N::receive( s, recv_iter<N>{ &v, j[u] }, itr, info, bak );
+				}
+			}
+			else
+			{
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					if( !ok[u] ) continue;
+					auto bak = make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE );
+					N::receive( s, recv_iter<N>{ &v, j[u] }, no_syn_iter{}, info, bak );
+				}
 			}
 		}
 	}
-	if( lane == 0 && ev ) atomicAdd( &v.stats->events, ev );
-	if constexpr( plastic )
+	block_add_stats<2>( v, { lane == 0 ? ev : 0ull, pev }, { ST_EVENTS, ST_PLASTIC } );
+}
+
+// ---- target-tiled spike delivery -----------------------------------------------------------
+constexpr int TILE_MAX = 1024; // targets per tile (LDS accumulators)
+constexpr int SPK_CHUNK = 256; // delivered spikes staged in LDS per pass
+
+// Per row r and tile t: offset (within the row) of the first target >= t * tile_w, for
+// t = 0 .. ntiles (the last entry is the row length). One wave per row, binary searches.
+static inline __global__ __launch_bounds__( 256 ) void k_pivots( dview v )
+{
+	std::int64_t const r = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
+	if( r >= v.N ) return;
+	std::int64_t const lo = v.row_ptr[r];
+	std::int32_t const len = std::int32_t( v.row_ptr[r + 1] - lo );
+	std::int32_t const * c = v.col + lo;
+	for( int t = lane_id(); t <= v.ntiles; t += 64 )
 	{
-		// plastic event counts are per lane
-		unsigned long long const tot = wave_sum( pev );
-		if( lane == 0 && tot ) atomicAdd( &v.stats->plastic_events, tot );
+		std::int64_t const x = std::int64_t( t ) * v.tile_w;
+		std::int32_t a = 0, b = len; // first index with c[index] >= x
+		while( a < b )
+		{
+			std::int32_t const m = ( a + b ) >> 1;
+			if( c[m] < x )
+				a = m + 1;
+			else
+				b = m;
+		}
+		v.pivots[r * ( v.ntiles + 1 ) + t] = a;
 	}
+}
+
+// Receive target inside a tile: reads see the neuron state, backend.atomic_add goes to the
+// tile's LDS accumulator of attribute I.
+template <typename T, int I, typename A, bool InMask>
+struct tile_proxy
+{
+	dview const * v;
+	std::int32_t j;
+	A * lds; // accumulator of attribute I for this tile, indexed by j - tile start
+	std::int32_t t0;
+	__device__ operator T() const { return ld<T>( v->na[I], j ); }
+};
+
+template <typename Neuron, bool FIXED>
+struct tile_iter
+{
+	dview const * v;
+	std::int32_t j;
+	std::int32_t t0;
+	unsigned long long * lds; // [nacc][TILE_MAX] 8-byte slots
+	template <int I>
+	__device__ auto get() const
+	{
+		using T = typename Neuron::template type<I>;
+		constexpr std::uint32_t mask = receive_mask_v<Neuron>;
+		constexpr int a = __builtin_popcount( mask & ( ( 1u << I ) - 1u ) );
+		using A = std::conditional_t<std::is_floating_point_v<T>, std::conditional_t<FIXED, long long, float>, T>;
+		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0>{ v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0 };
+	}
+	__device__ uint_ id() const { return uint_( j ); }
+};
+
+template <bool FIXED>
+struct tile_backend : dev_backend<FIXED>
+{
+	using dev_backend<FIXED>::atomic_add;
+	template <typename T, int I, typename A, bool InMask>
+	__device__ void atomic_add( tile_proxy<T, I, A, InMask> p, typename util::identity<T>::type x ) const
+	{
+		static_assert( InMask, "receive() adds to a neuron attribute that the model's receive_mask does not list" );
+		A * dst = p.lds + ( p.j - p.t0 );
+		if constexpr( std::is_floating_point_v<T> && FIXED )
+			atomicAdd( reinterpret_cast<unsigned long long *>( dst ),
+			           (unsigned long long)( long long )__builtin_rint( double( x ) * double( 1ull << this->shift ) ) );
+		else
+			atomicAdd( dst, A( x ) );
+	}
+};
+
+// Delivers the spikes of step T - D, one workgroup per tile of tile_w target neurons:
+// the workgroup stages a chunk of spikes (id, segment of the row inside the tile, and for
+// plastic models the row's lazy state) in LDS, forms the segments' prefix sums, and walks
+// the concatenated segments as one flat index space (4 entries per thread in flight, the
+// owning spike found by binary search in LDS). receive() accumulates into LDS with LDS
+// atomics; at the end the tile's sums go to the global accumulators with plain stores:
+// one workgroup owns each tile, so no global atomics are needed.
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
+{
+	using N = typename Model::neuron;
+	using S = typename Model::synapse;
+	constexpr bool plastic = is_plastic_v<Model>;
+	constexpr std::uint32_t rmask = receive_mask_v<N>;
+	constexpr int nacc = __builtin_popcount( rmask ) > 0 ? __builtin_popcount( rmask ) : 1;
+	constexpr int U = 4;
+
+	__shared__ unsigned long long acc[nacc * TILE_MAX];
+	__shared__ std::int64_t seg_beg[SPK_CHUNK];
+	__shared__ std::int32_t seg_pre[SPK_CHUNK + 1];
+	__shared__ std::int32_t seg_src[SPK_CHUNK];
+	__shared__ std::int32_t seg_synced[plastic ? SPK_CHUNK : 1];
+	__shared__ std::int32_t seg_pend[plastic ? SPK_CHUNK : 1];
+	__shared__ std::int32_t wsum[4];
+
+	std::int64_t const T = v.ctl->step;
+	int const D1 = v.D + 1;
+	int const slot = int( ( T + 1 ) % D1 ); // ring entry of step T - D
+	std::int32_t const nspk = T < v.D ? 0 : std::int32_t( v.counts[slot] );
+	std::int32_t const * spk = v.ring + std::int64_t( slot ) * v.N;
+	float const dt = step_dt( v );
+	snn_info const info = info_of( v );
+	int const tid = threadIdx.x;
+	unsigned long long ev = 0, pev = 0;
+
+	for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x )
+	{
+		std::int32_t const t0 = tile * v.tile_w;
+		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
+		for( int x = tid; x < nacc * TILE_MAX; x += 256 ) acc[x] = 0ull;
+		__syncthreads();
+
+		for( std::int32_t c0 = 0; c0 < nspk; c0 += SPK_CHUNK )
+		{
+			std::int32_t const cnt = ( nspk - c0 ) < SPK_CHUNK ? ( nspk - c0 ) : SPK_CHUNK;
+			std::int32_t len = 0;
+			if( tid < cnt )
+			{
+				std::int32_t const s = spk[c0 + tid];
+				std::int32_t const * pv = v.pivots + std::int64_t( s ) * ( v.ntiles + 1 ) + tile;
+				std::int32_t const a = pv[0], b = pv[1];
+				seg_beg[tid] = v.row_ptr[s] + a;
+				seg_src[tid] = s;
+				len = b - a;
+				if constexpr( plastic )
+				{
+					seg_synced[tid] = v.synced[s];
+					seg_pend[tid] = v.pend[s];
+				}
+			}
+			// exclusive block scan of the segment lengths
+			std::uint64_t const incl = wave_inclusive_scan( std::uint64_t( len ) );
+			if( lane_id() == 63 ) wsum[tid >> 6] = std::int32_t( incl );
+			__syncthreads();
+			std::int32_t wb = 0;
+			for( int w = 0; w < ( tid >> 6 ); w++ ) wb += wsum[w];
+			if( tid < cnt ) seg_pre[tid] = wb + std::int32_t( incl ) - len;
+			if( tid == 255 ) seg_pre[cnt] = wb + std::int32_t( incl );
+			__syncthreads();
+			std::int32_t const total = seg_pre[cnt];
+
+			for( std::int32_t f0 = 0; f0 < total; f0 += 256 * U )
+			{
+				std::int32_t kk[U];
+				std::int64_t e[U];
+				std::int32_t j[U];
+				bool ok[U];
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					std::int32_t const f = f0 + tid + 256 * u;
+					ok[u] = f < total;
+					std::int32_t a = 0, b = cnt - 1; // largest k with seg_pre[k] <= f
+					while( a < b )
+					{
+						std::int32_t const m = ( a + b + 1 ) >> 1;
+						if( seg_pre[m] <= f )
+							a = m;
+						else
+							b = m - 1;
+					}
+					kk[u] = a;
+					e[u] = seg_beg[a] + ( f - seg_pre[a] );
+					j[u] = ok[u] ? v.col[e[u]] : 0;
+				}
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					if( !ok[u] ) continue;
+					std::int32_t const s = seg_src[kk[u]];
+					tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
+					tile_iter<N, FIXED> dst{ &v, j[u], t0, acc };
+					if constexpr( plastic )
+					{
+						typename S::tuple_t sy;
+						load_tuple<S>( sy, v.sa, e[u] );
+						reg_iter<typename S::tuple_t> itr{ sy, std::uint32_t( e[u] ) };
+						if( S::plastic( s, j[u], info ) )
+						{
+							std::int32_t const synced = seg_synced[kk[u]], pend = seg_pend[kk[u]];
+							bool changed;
+							if constexpr( has_neuron_traces_v<S> )
+								changed = replay_traced<S>( itr, v.hist[j[u]], std::int32_t( T - 1 ), synced, pend, s, j[u], v, info, bak );
+							else
+							{
+								replay<S>( itr, v.hist[j[u]], std::int32_t( T - 1 ), synced, pend, dt, info, bak );
+								changed = true;
+							}
+							if( changed )
+							{
+								store_tuple<S>( sy, v.sa, e[u] );
+								pev++;
+							}
+						}
+						N::receive( s, dst, itr, info, bak );
+					}
+					else
+						N::receive( s, dst, no_syn_iter{}, info, bak );
+				}
+			}
+			if( tid == 0 ) ev += std::uint64_t( total );
+			__syncthreads(); // segments are overwritten by the next chunk
+		}
+
+		// the tile's sums -> global accumulators (this workgroup owns them; they are zero here)
+		util::for_n<N::size>( [&]( auto I ) {
+			if constexpr( ( ( rmask >> I ) & 1u ) != 0 )
+			{
+				using T = typename N::template type<I>;
+				using A = std::conditional_t<std::is_floating_point_v<T>, std::conditional_t<FIXED, long long, float>, T>;
+				constexpr int a = __builtin_popcount( rmask & ( ( 1u << I ) - 1u ) );
+				A const * src = reinterpret_cast<A const *>( acc + a * TILE_MAX );
+				for( int x = tid; x < tw; x += 256 )
+				{
+					A const val = src[x];
+					if( val != A( 0 ) ) st<A>( v.acc[I], t0 + x, val );
+				}
+			}
+		} );
+		__syncthreads();
+	}
+	block_add_stats<2>( v, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
 }
 
 template <typename Model, bool FIXED>
@@ -356,6 +675,13 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 	{
 		v.counts[( T + 1 ) % D1] = 0; // consumed by this step's receive; refilled at T + 1
 		if constexpr( plastic ) v.fl_count[( T + 1 ) & 1] = 0;
+		// next step's clock: exactly kahan_sum::add on the host
+		ctl_t const c = *v.ctl;
+		float const y = v.DT - c.c;
+		float const t = c.sum + y;
+		v.ctl_next->c = ( t - c.sum ) - y;
+		v.ctl_next->sum = t;
+		v.ctl_next->step = c.step + 1;
 	}
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
@@ -393,17 +719,32 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 
 		if constexpr( plastic )
 		{
+			using S = typename Model::synapse;
 			unsigned long long h = v.hist[i];
 			std::int32_t synced = v.synced[i], pend = v.pend[i];
 			bool dirty = false;
-			if( ( h >> ( v.D - 1 ) ) & 1ull ) // this neuron's spike was delivered this step
+			bool const arrived = ( h >> ( v.D - 1 ) ) & 1ull; // this neuron's spike reached its synapses now
+			if( arrived )
 			{
 				synced = std::int32_t( T - 1 );
 				pend = 1;
 				dirty = true;
 			}
 			v.hist[i] = ( h << 1 ) | ( spiked ? 1ull : 0ull );
-			if( v.hasp[i] && T - synced >= FLUSH_AGE )
+			if constexpr( has_neuron_traces_v<S> )
+			{
+				float const d = S::trace_decay( dt );
+				float const x = fmaf( v.xtr[i], d, arrived ? 1.0f : 0.0f );
+				float const y = fmaf( v.ytr[i], d, spiked ? 1.0f : 0.0f );
+				v.xtr[i] = x;
+				v.ytr[i] = y;
+				std::int64_t const slot = T & ( HISTORY - 1 );
+				v.xh[slot * v.N + i] = x;
+				v.yh[slot * v.N + i] = y;
+			}
+			// Rows are flushed before their window overflows; a row never processed yet gets its
+			// first flush at step (i mod FLUSH_AGE), which spreads flushes evenly over steps.
+			if( v.hasp[i] && ( T - synced >= FLUSH_AGE || ( synced < 0 && T >= i % FLUSH_AGE ) ) )
 			{
 				flush = true;
 				fl_synced = synced;
@@ -420,44 +761,53 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 		}
 	}
 
-	// spike compaction: one atomic per wave
+	// Spike compaction: ballot + popcount per wave, prefix over the block's waves in LDS, one
+	// atomic per block on the step's counter, mbcnt gives each spiking lane its slot.
+	__shared__ unsigned wcount[4][2];
+	__shared__ unsigned bbase[2];
+	int const w = threadIdx.x >> 6;
 	std::uint64_t const m = ballot( spiked );
-	if( m )
+	std::uint64_t const fm = plastic ? ballot( flush ) : 0ull;
+	if( lane_id() == 0 )
 	{
-		unsigned base = 0;
-		if( lane_id() == 0 )
-		{
-			base = atomicAdd( &v.counts[wslot], unsigned( __popcll( m ) ) );
-			atomicAdd( &v.stats->spikes, (unsigned long long)__popcll( m ) );
-		}
-		base = __builtin_amdgcn_readfirstlane( base );
-		if( spiked ) v.ring[std::int64_t( wslot ) * v.N + base + lanes_below( m )] = std::int32_t( i );
+		wcount[w][0] = unsigned( __popcll( m ) );
+		wcount[w][1] = unsigned( __popcll( fm ) );
 	}
+	__syncthreads();
+	if( threadIdx.x == 0 )
+	{
+		unsigned t0 = 0, t1 = 0;
+		for( int k = 0; k < 4; k++ )
+		{
+			unsigned const a = wcount[k][0], b = wcount[k][1];
+			wcount[k][0] = t0;
+			wcount[k][1] = t1;
+			t0 += a;
+			t1 += b;
+		}
+		bbase[0] = t0 ? atomicAdd( &v.counts[wslot], t0 ) : 0u;
+		if constexpr( plastic ) bbase[1] = t1 ? atomicAdd( &v.fl_count[T & 1], t1 ) : 0u;
+		if( t0 ) v.bstats[std::int64_t( blockIdx.x ) * NSTAT + ST_SPIKES] += t0;
+	}
+	__syncthreads();
+	if( spiked ) v.ring[std::int64_t( wslot ) * v.N + bbase[0] + wcount[w][0] + lanes_below( m )] = std::int32_t( i );
 	if constexpr( plastic )
 	{
-		std::uint64_t const fm = ballot( flush );
-		if( fm )
+		if( flush )
 		{
-			unsigned base = 0;
-			if( lane_id() == 0 ) base = atomicAdd( &v.fl_count[T & 1], unsigned( __popcll( fm ) ) );
-			base = __builtin_amdgcn_readfirstlane( base );
-			if( flush )
-			{
-				unsigned const o = base + lanes_below( fm );
-				v.fl_id[o] = std::int32_t( i );
-				v.fl_synced[o] = fl_synced;
-				v.fl_pend[o] = fl_pend;
-			}
+			unsigned const o = bbase[1] + wcount[w][1] + lanes_below( fm );
+			v.fl_id[o] = std::int32_t( i );
+			v.fl_synced[o] = fl_synced;
+			v.fl_pend[o] = fl_pend;
 		}
 	}
-	else
-		finish_step( v );
 }
 
 template <typename Model, bool FIXED>
 __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 {
 	using S = typename Model::synapse;
+	constexpr int U = 4; // slots per lane per iteration: independent loads in flight
 	std::int64_t const T = v.ctl->step;
 	std::int64_t const nitems = std::int64_t( v.fl_count[T & 1] ) * v.fparts;
 	float const dt = step_dt( v );
@@ -465,7 +815,7 @@ __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 	int const lane = lane_id();
 	std::int64_t const wave0 = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
 	std::int64_t const nwaves = std::int64_t( gridDim.x ) * ( blockDim.x >> 6 );
-	unsigned long long touched = 0;
+	unsigned long long touched = 0, slots = 0, rows = 0;
 	for( std::int64_t it = wave0; it < nitems; it += nwaves )
 	{
 		std::int64_t const f = it / v.fparts;
@@ -473,27 +823,55 @@ __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 		std::int32_t const s = uniform( v.fl_id[f] );
 		std::int32_t const synced = uniform( v.fl_synced[f] );
 		std::int32_t const pend = uniform( v.fl_pend[f] );
-		std::int64_t const lo = v.row_ptr[s], hi = v.row_ptr[s + 1];
-		std::int64_t const chunk = ( ( hi - lo + v.fparts - 1 ) / v.fparts + 63 ) & ~std::int64_t( 63 );
+		std::int64_t const r0 = v.row_ptr[s];
+		std::int64_t const lo = r0 + v.pl_lo[s], hi = r0 + v.pl_hi[s];
+		std::int64_t const chunk = ( ( hi - lo + v.fparts - 1 ) / v.fparts + 255 ) & ~std::int64_t( 255 );
 		std::int64_t const beg = lo + p * chunk;
 		std::int64_t const end = beg + chunk < hi ? beg + chunk : hi;
-		for( std::int64_t e = beg + lane; e < end; e += 64 )
+		unsigned long long const wmask = window_mask( std::int32_t( T ) - synced );
+		for( std::int64_t base = beg; base < end; base += 64 * U )
 		{
-			std::int32_t const j = v.col[e];
-			if( !S::plastic( s, j, info ) ) continue;
-			typename S::tuple_t sy;
-			load_tuple<S>( sy, v.sa, e );
-			auto bak = make_backend<FIXED>(
-			    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
-			replay<S>( reg_iter<typename S::tuple_t>{ sy, std::uint32_t( e ) }, v.hist[j], std::int32_t( T ), synced, pend, dt, info, bak );
-			store_tuple<S>( sy, v.sa, e );
-			touched++;
+			std::int32_t j[U];
+			bool ok[U];
+#pragma unroll
+			for( int u = 0; u < U; u++ )
+			{
+				std::int64_t const e = base + lane + 64 * u;
+				ok[u] = e < end;
+				j[u] = ok[u] ? v.col[e] : 0;
+			}
+#pragma unroll
+			for( int u = 0; u < U; u++ ) ok[u] = ok[u] && S::plastic( s, j[u], info );
+			unsigned long long hj[U];
+#pragma unroll
+			for( int u = 0; u < U; u++ ) hj[u] = ok[u] ? v.hist[j[u]] : 0ull;
+#pragma unroll
+			for( int u = 0; u < U; u++ )
+			{
+				if( !ok[u] ) continue;
+				if constexpr( has_neuron_traces_v<S> )
+					if( !pend && !( hj[u] & wmask ) ) continue; // no event for this synapse in the window
+				std::int64_t const e = base + lane + 64 * u;
+				typename S::tuple_t sy;
+				load_tuple<S>( sy, v.sa, e );
+				auto bak = make_backend<FIXED>(
+				    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
+				reg_iter<typename S::tuple_t> itr{ sy, std::uint32_t( e ) };
+				if constexpr( has_neuron_traces_v<S> )
+					replay_traced<S>( itr, hj[u], std::int32_t( T ), synced, pend, s, j[u], v, info, bak );
+				else
+					replay<S>( itr, hj[u], std::int32_t( T ), synced, pend, dt, info, bak );
+				store_tuple<S>( sy, v.sa, e );
+				touched++;
+			}
 		}
-		if( lane == 0 && p == 0 ) atomicAdd( &v.stats->flush_rows, 1ull );
+		if( lane == 0 )
+		{
+			if( end > beg ) slots += std::uint64_t( end - beg );
+			if( p == 0 ) rows++;
+		}
 	}
-	unsigned long long const tot = wave_sum( touched );
-	if( lane == 0 && tot ) atomicAdd( &v.stats->flush_synapses, tot );
-	finish_step( v );
+	block_add_stats<3>( v, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
 // Brings every row with plastic synapses up to date through step `now` (= last completed
@@ -506,15 +884,24 @@ __global__ __launch_bounds__( 256 ) void k_sync_all( dview v, std::int32_t now, 
 	if( r >= v.N || !v.hasp[r] ) return;
 	snn_info const info = info_of( v );
 	std::int32_t const synced = v.synced[r], pend = v.pend[r];
-	for( std::int64_t e = v.row_ptr[r] + lane_id(); e < v.row_ptr[r + 1]; e += 64 )
+	std::int64_t const r0 = v.row_ptr[r];
+	for( std::int64_t e = r0 + v.pl_lo[r] + lane_id(); e < r0 + v.pl_hi[r]; e += 64 )
 	{
 		std::int32_t const j = v.col[e];
 		if( !S::plastic( std::int32_t( r ), j, info ) ) continue;
 		typename S::tuple_t sy;
 		load_tuple<S>( sy, v.sa, e );
 		auto bak = make_backend<false>( v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( now ), util::TAG_SYNAPSE_UPDATE );
-		replay<S>( reg_iter<typename S::tuple_t>{ sy, std::uint32_t( e ) }, v.hist[j], now, synced, pend, dt, info, bak );
-		store_tuple<S>( sy, v.sa, e );
+		reg_iter<typename S::tuple_t> it{ sy, std::uint32_t( e ) };
+		if constexpr( has_neuron_traces_v<S> )
+		{
+			if( replay_traced<S>( it, v.hist[j], now, synced, pend, std::int32_t( r ), j, v, info, bak ) ) store_tuple<S>( sy, v.sa, e );
+		}
+		else
+		{
+			replay<S>( it, v.hist[j], now, synced, pend, dt, info, bak );
+			store_tuple<S>( sy, v.sa, e );
+		}
 	}
 	if( lane_id() == 0 )
 	{

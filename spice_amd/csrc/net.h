@@ -76,6 +76,7 @@ class hip_net final : public net_iface
 	using N = typename Model::neuron;
 	using S = typename Model::synapse;
 	static constexpr bool plastic = is_plastic_v<Model>;
+	static constexpr bool traced = plastic && has_neuron_traces_v<S>;
 	static constexpr int graph_steps = 32; // steps per captured graph
 
 public:
@@ -104,7 +105,7 @@ public:
 	// ---- simulation ----
 	void step( std::int32_t ** dspk, std::uint32_t ** dcnt ) override
 	{
-		launch_step();
+		launch_step( _step );
 		std::int64_t const t = _step++;
 		_clock.add( _p.dt );
 		int const slot = int( t % ( _p.delay + 1 ) );
@@ -116,17 +117,27 @@ public:
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
 		std::int64_t done = 0;
-		if( n >= graph_steps )
-		{
-			if( !_exec ) capture();
-			for( ; done + graph_steps <= n; done += graph_steps ) SPICE_HIP( hipGraphLaunch( _exec, _s ) );
-		}
-		for( ; done < n; done++ ) launch_step();
-		for( std::int64_t i = 0; i < n; i++ )
-		{
+		auto eager = [&] {
+			launch_step( _step );
 			_step++;
 			_clock.add( _p.dt );
+			done++;
+		};
+		if( ( _step & 1 ) && done < n ) eager(); // graphs start at an even step
+		if( n - done >= graph_steps )
+		{
+			if( !_exec ) capture();
+			for( ; done + graph_steps <= n; done += graph_steps )
+			{
+				SPICE_HIP( hipGraphLaunch( _exec, _s ) );
+				for( int i = 0; i < graph_steps; i++ )
+				{
+					_step++;
+					_clock.add( _p.dt );
+				}
+			}
 		}
+		while( done < n ) eager();
 	}
 
 	// Runs n steps eagerly with HIP events around every kernel on the net's stream and adds
@@ -146,9 +157,9 @@ public:
 				{
 					hipEvent_t * e = &ev[i * ev_per_step];
 					if( fixed() )
-						launch_step_events<true>( e );
+						launch_step_events<true>( e, _vs[_step & 1] );
 					else
-						launch_step_events<false>( e );
+						launch_step_events<false>( e, _vs[_step & 1] );
 					_step++;
 					_clock.add( _p.dt );
 				}
@@ -216,8 +227,8 @@ public:
 	{
 		if( step < 0 || step > INT_MAX - 2 ) throw std::invalid_argument( "invalid step index" );
 		synchronize();
-		ctl_t ctl{ step, sum, c, 0, 0 };
-		SPICE_HIP( hipMemcpy( _ctl.data(), &ctl, sizeof( ctl ), hipMemcpyHostToDevice ) );
+		ctl_t ctl{ step, sum, c };
+		SPICE_HIP( hipMemcpy( _ctl.data() + ( step & 1 ), &ctl, sizeof( ctl ), hipMemcpyHostToDevice ) );
 		SPICE_HIP( hipMemset( _counts.data(), 0, _counts.bytes() ) );
 		if constexpr( plastic )
 		{
@@ -226,6 +237,8 @@ public:
 			SPICE_HIP( hipMemcpy( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice ) );
 			SPICE_HIP( hipMemset( _pend.data(), 0, _pend.bytes() ) );
 			SPICE_HIP( hipMemset( _fl_count.data(), 0, _fl_count.bytes() ) );
+			if constexpr( traced )
+				for( auto * b : { &_xtr, &_ytr, &_xh, &_yh } ) SPICE_HIP( hipMemset( b->data(), 0, b->bytes() ) );
 		}
 		_step = step;
 		_clock.set( sum, c );
@@ -233,15 +246,19 @@ public:
 
 	void get_stats( spice_stats * out ) override
 	{
-		stats_t s;
-		SPICE_HIP( hipMemcpyAsync( &s, _stats.data(), sizeof( s ), hipMemcpyDeviceToHost, _s ) );
+		std::vector<unsigned long long> b( _bstats.size() );
+		SPICE_HIP( hipMemcpyAsync( b.data(), _bstats.data(), _bstats.bytes(), hipMemcpyDeviceToHost, _s ) );
 		SPICE_HIP( hipStreamSynchronize( _s ) );
-		*out = { s.steps, s.spikes, s.events, s.plastic_events, s.flush_rows, s.flush_synapses };
+		unsigned long long t[NSTAT] = {};
+		for( std::size_t i = 0; i < b.size(); i++ ) t[i % NSTAT] += b[i];
+		*out = { std::uint64_t( _step - _stats_step0 ), t[ST_SPIKES], t[ST_EVENTS], t[ST_PLASTIC], t[ST_FLUSH_ROWS],
+		         t[ST_FLUSH_SYN], t[ST_FLUSH_SLOTS] };
 	}
 	void reset_stats() override
 	{
-		SPICE_HIP( hipMemsetAsync( _stats.data(), 0, _stats.bytes(), _s ) );
+		SPICE_HIP( hipMemsetAsync( _bstats.data(), 0, _bstats.bytes(), _s ) );
 		synchronize();
+		_stats_step0 = _step;
 	}
 
 	// ---- export / import ----
@@ -336,12 +353,17 @@ private:
 	dbuf<unsigned long long> _hist;
 	dbuf<std::int32_t> _synced, _pend, _fl_id, _fl_synced, _fl_pend;
 	dbuf<std::uint8_t> _hasp;
+	dbuf<std::int32_t> _pl_lo, _pl_hi;
 	dbuf<std::uint32_t> _fl_count;
+	dbuf<float> _xtr, _ytr, _xh, _yh;
 	dbuf<ctl_t> _ctl;
-	dbuf<stats_t> _stats;
-	dview _v{};
+	dbuf<unsigned long long> _bstats;
+	std::int64_t _stats_step0 = 0;
+	dview _v{};      // step views: _vs[p] reads clock slot p
+	dview _vs[2]{};
 
-	int _grid_receive = 0, _grid_update = 0, _grid_flush = 0;
+	int _grid_receive = 0, _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
+	dbuf<std::int32_t> _pivots;
 	std::int64_t _step = 0;
 	util::kahan_sum<float> _clock;
 
@@ -387,6 +409,8 @@ private:
 			_synced.resize( n );
 			_pend.resize( n );
 			_hasp.resize( n );
+			_pl_lo.resize( n );
+			_pl_hi.resize( n );
 			_fl_id.resize( n );
 			_fl_synced.resize( n );
 			_fl_pend.resize( n );
@@ -394,14 +418,20 @@ private:
 			SPICE_HIP( hipMemsetAsync( _hist.data(), 0, _hist.bytes(), _s ) );
 			SPICE_HIP( hipMemsetAsync( _pend.data(), 0, _pend.bytes(), _s ) );
 			SPICE_HIP( hipMemsetAsync( _fl_count.data(), 0, _fl_count.bytes(), _s ) );
+			if constexpr( traced )
+			{
+				_xtr.resize( n );
+				_ytr.resize( n );
+				_xh.resize( std::size_t( HISTORY ) * n );
+				_yh.resize( std::size_t( HISTORY ) * n );
+				for( auto * b : { &_xtr, &_ytr, &_xh, &_yh } ) SPICE_HIP( hipMemsetAsync( b->data(), 0, b->bytes(), _s ) );
+			}
 			std::vector<std::int32_t> synced( n, -1 );
 			SPICE_HIP( hipMemcpyAsync( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice, _s ) );
 			SPICE_HIP( hipStreamSynchronize( _s ) ); // `synced` is a host temporary
 		}
-		_ctl.resize( 1 );
-		_stats.resize( 1 );
+		_ctl.resize( 2 );
 		SPICE_HIP( hipMemsetAsync( _ctl.data(), 0, _ctl.bytes(), _s ) );
-		SPICE_HIP( hipMemsetAsync( _stats.data(), 0, _stats.bytes(), _s ) );
 
 		dview & v = _v;
 		v.N = std::int32_t( n );
@@ -425,26 +455,48 @@ private:
 		v.synced = _synced.data();
 		v.pend = _pend.data();
 		v.hasp = _hasp.data();
+		v.pl_lo = _pl_lo.data();
+		v.pl_hi = _pl_hi.data();
 		v.fl_id = _fl_id.data();
 		v.fl_synced = _fl_synced.data();
 		v.fl_pend = _fl_pend.data();
 		v.fl_count = _fl_count.data();
+		v.xtr = _xtr.data();
+		v.ytr = _ytr.data();
+		v.xh = _xh.data();
+		v.yh = _yh.data();
 		v.ctl = _ctl.data();
-		v.stats = _stats.data();
+		v.ctl_next = _ctl.data() + 1;
 
 		// Launch shapes. Rows longer than ~1k targets are split so one delivered spike
 		// occupies several waves; grids are persistent (work is read from device memory).
 		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
 		v.parts = std::max( 1, int( mean_row / 1024.0 + 0.5 ) );
-		v.fparts = v.parts;
+		v.fparts = std::max( 1, int( mean_row / 2048.0 + 0.5 ) );
 		int const cus = device_cu_count();
 		_grid_update = int( ( n + 255 ) / 256 );
-		_grid_receive = int( std::min<std::int64_t>( 4 * cus, std::max<std::int64_t>( 1, ( n * v.parts + 63 ) / 64 ) ) );
-		_grid_flush = _grid_receive;
+		// persistent grids: up to 8 waves per SIMD (the kernels use few VGPRs)
+		_grid_receive = int( std::min<std::int64_t>( 8 * cus, std::max<std::int64_t>( 1, ( n * v.parts + 3 ) / 4 ) ) );
+		_grid_flush = int( std::min<std::int64_t>( 8 * cus, std::max<std::int64_t>( 1, ( n * v.fparts + 3 ) / 4 ) ) );
+		// target tiles for spike delivery: ~2 per CU, 256 .. TILE_MAX targets each
+		v.tile_w = int( std::min<std::int64_t>( TILE_MAX, std::max<std::int64_t>( 256, ( ( n + 2 * cus - 1 ) / ( 2 * cus ) + 63 ) / 64 * 64 ) ) );
+		v.ntiles = int( ( n + v.tile_w - 1 ) / v.tile_w );
+		_pivots.resize( std::size_t( n ) * ( v.ntiles + 1 ) );
+		v.pivots = _pivots.data();
+		_grid_tiles = std::min( v.ntiles, 16 * cus );
+		int const max_blocks = std::max( { _grid_update, _grid_receive, _grid_flush, _grid_tiles } );
+		_bstats.resize( std::size_t( max_blocks ) * NSTAT );
+		SPICE_HIP( hipMemsetAsync( _bstats.data(), 0, _bstats.bytes(), _s ) );
+		v.bstats = _bstats.data();
 	}
 
 	void initialize()
 	{
+		_vs[0] = _v;
+		_vs[1] = _v;
+		_vs[1].ctl = _ctl.data() + 1;
+		_vs[1].ctl_next = _ctl.data();
+		SPICE_LAUNCH( k_pivots<<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
 		SPICE_LAUNCH( k_init_neurons<Model><<<_grid_update, 256, 0, _s>>>( _v ) );
 		if constexpr( plastic )
 		{
@@ -453,31 +505,32 @@ private:
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
-	void launch_step()
+	// Enqueues the kernels of step `t` (its parity selects the clock slot).
+	void launch_step( std::int64_t t )
 	{
 		if( fixed() )
-			launch_step_t<true>();
+			launch_step_t<true>( _vs[t & 1] );
 		else
-			launch_step_t<false>();
+			launch_step_t<false>( _vs[t & 1] );
 	}
 
 	template <bool F>
-	void launch_step_t()
+	void launch_step_t( dview const & v )
 	{
-		SPICE_LAUNCH( k_receive<Model, F><<<_grid_receive, 256, 0, _s>>>( _v ) );
-		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( _v ) );
-		if constexpr( plastic ) SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( _v ) );
+		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, 256, 0, _s>>>( v ) );
+		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
+		if constexpr( plastic ) SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( v ) );
 	}
 
 	template <bool F>
-	void launch_step_events( hipEvent_t * e )
+	void launch_step_events( hipEvent_t * e, dview const & v )
 	{
 		SPICE_HIP( hipEventRecord( e[0], _s ) );
-		SPICE_LAUNCH( k_receive<Model, F><<<_grid_receive, 256, 0, _s>>>( _v ) );
+		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, 256, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[1], _s ) );
-		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( _v ) );
+		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[2], _s ) );
-		if constexpr( plastic ) SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( _v ) );
+		if constexpr( plastic ) SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[3], _s ) );
 	}
 
@@ -487,7 +540,7 @@ private:
 		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
 		try
 		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step();
+			for( int i = 0; i < graph_steps; i++ ) launch_step( i ); // parity 0, 1, 0, ...
 		}
 		catch( ... )
 		{

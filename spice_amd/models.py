@@ -9,8 +9,7 @@ MODELS = {
                                [("W", np.float32), ("Zpre", np.float32), ("Zpost", np.float32)]),
     "synth": (3, [("N", np.int32)], []),
     "brunel_3pop": (4, [("V", np.float32), ("Twait", np.int32)], []),
-    "brunel_3pop_stdp": (5, [("V", np.float32), ("Twait", np.int32)],
-                         [("W", np.float32), ("Zpre", np.float32), ("Zpost", np.float32)]),
+    "brunel_3pop_stdp": (5, [("V", np.float32), ("Twait", np.int32)], [("W", np.float32)]),
 }
 
 PLASTIC = {name for name, (_, _, syn) in MODELS.items() if syn}

@@ -94,10 +94,11 @@ def test_lif_refractory_period():
     assert nspk > 100
 
 
-def test_stdp_traces_are_neuron_local_and_match_spike_trains():
-    """Plastic-model invariants: every synapse's pre trace equals the decayed sum over its
-    source's spike arrivals, every post trace the decayed sum over its target's spikes, so
-    traces depend only on the neuron; weights stay in [0, wmax]; non-plastic weights never move."""
+def test_stdp_traces_match_spike_trains_and_weights_stay_bounded():
+    """Trace-synapse model: the pre trace of every neuron equals the decayed sum over its
+    spike arrivals (emission + delay) and the post trace the decayed sum over its spikes
+    (closed form from the emitted spike trains); weights stay in [0, wmax]; non-plastic
+    weights never move; plasticity moves some weights."""
     n = 1000
     npois = n // 5
     ne = (n - npois) * 4 // 5
@@ -108,21 +109,59 @@ def test_stdp_traces_are_neuron_local_and_match_spike_trains():
     spk = np.zeros((steps, n), bool)
     for t in range(steps):
         spk[t, h.step(True)] = True
+    x, y = h.traces()
+    d = 1 - DT / 0.02
+    decay = d ** (steps - 1 - np.arange(steps))
+    arr = np.zeros_like(spk)
+    arr[D:] = spk[:-D]
+    np.testing.assert_allclose(x, (arr * decay[:, None]).sum(0), rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(y, (spk * decay[:, None]).sum(0), rtol=1e-4, atol=1e-6)
     rp, col = h.csr()
-    syn = h.synapses(ell=False)
+    w = h.synapses(ell=False)["W"]
     src = np.repeat(np.arange(n), np.diff(rp))
     plastic = (src >= npois) & (src < npois + ne) & (col >= npois) & (col < npois + ne)
-    d = 1 - DT / 0.02
-    age = steps - 1 - np.arange(steps)
-    decay = d ** age
-    pre = np.zeros(n)
-    arr = np.zeros_like(spk)
-    arr[D:] = spk[:-D]  # arrivals
-    pre = (arr * decay[:, None]).sum(0)
-    post = (spk * decay[:, None]).sum(0)
-    np.testing.assert_allclose(syn["Zpre"][plastic], pre[src[plastic]], rtol=1e-4, atol=1e-6)
-    np.testing.assert_allclose(syn["Zpost"][plastic], post[col[plastic]], rtol=1e-4, atol=1e-6)
-    assert np.all((syn["W"][plastic] >= 0) & (syn["W"][plastic] <= np.float32(3e-4)))
+    assert np.all((w[plastic] >= 0) & (w[plastic] <= np.float32(3e-4)))
     w0 = np.where(src < npois + ne, np.float32(1e-4), np.float32(-5e-4))
-    np.testing.assert_array_equal(syn["W"][~plastic], w0[~plastic])
-    assert np.any(syn["W"][plastic] != np.float32(1e-4))  # plasticity did something
+    np.testing.assert_array_equal(w[~plastic], w0[~plastic])
+    assert np.any(w[plastic] != np.float32(1e-4))
+
+
+def test_stdp_weight_change_single_pair_closed_form():
+    """Additive pair rule on a tiny net: every plastic weight equals its initial value plus
+    A+ wmax times the sum over post spikes of the pre trace minus A- wmax times the sum over
+    arrivals of the post trace, clipped to [0, wmax] (no clipping occurs at these rates, so
+    the unclipped closed form applies), with traces taken from the spike trains."""
+    n = 400
+    npois = n // 5
+    ne = (n - npois) * 4 // 5
+    lay = ([npois, ne, n - npois - ne], [(0, 1, 0.3), (1, 1, 0.2), (2, 1, 0.1)])
+    D = 4
+    h = O.CpuSnn("brunel_3pop_stdp", lay, DT, D, seed=8)
+    steps = 300
+    spk = np.zeros((steps, n), bool)
+    for t in range(steps):
+        spk[t, h.step(True)] = True
+    arr = np.zeros_like(spk)
+    arr[D:] = spk[:-D]
+    d = 1 - DT / 0.02
+    # traces after each step t: x_t = sum_{u<=t} arr[u] d^(t-u)
+    X = np.zeros((steps, n))
+    Y = np.zeros((steps, n))
+    xs = np.zeros(n)
+    ys = np.zeros(n)
+    for t in range(steps):
+        xs = xs * d + arr[t]
+        ys = ys * d + spk[t]
+        X[t], Y[t] = xs, ys
+    rp, col = h.csr()
+    w = h.synapses(ell=False)["W"].astype(np.float64)
+    src = np.repeat(np.arange(n), np.diff(rp))
+    plastic = (src >= npois) & (src < npois + ne) & (col >= npois) & (col < npois + ne)
+    Ap, Am = 0.01 * 3e-4, 0.0105 * 3e-4
+    s_idx, d_idx = src[plastic], col[plastic]
+    pot = (spk[:, d_idx] * X[:, s_idx]).sum(0)
+    dep = (arr[:, s_idx] * Y[:, d_idx]).sum(0)
+    expect = 1e-4 + Ap * pot - Am * dep
+    assert np.all((expect > 0) & (expect < 3e-4))
+    np.testing.assert_allclose(w[plastic], expect, rtol=1e-5, atol=1e-10)
+    assert np.any(pot > 0) and np.any(dep > 0)
