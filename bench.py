@@ -102,6 +102,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=1_000)
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--flags", type=int, default=0, help="spice_flag launch-shape bits (see include/spice_hip.h)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -116,7 +117,7 @@ def main():
     dt = 1e-4
 
     t0 = time.perf_counter()
-    net = Snn(model, Layout(pops, conns), dt, delay, seed=args.seed)
+    net = Snn(model, Layout(pops, conns), dt, delay, seed=args.seed, flags=args.flags)
     net.synchronize()
     setup_s = time.perf_counter() - t0
     E = net.num_edges()

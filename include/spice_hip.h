@@ -54,7 +54,10 @@ enum spice_flag
 	/* Sum receive() contributions per target attribute as int64 fixed point (units of
 	 * 2^-40) instead of fp32 atomics. Integer addition is associative, so results do not
 	 * depend on atomic order and match the CPU checker's fixed-point mode bit for bit. */
-	SPICE_FLAG_FIXED_POINT = 1u << 0
+	SPICE_FLAG_FIXED_POINT = 1u << 0,
+	/* Launch-shape choice (same results): bring queued plastic rows up to date with the
+	 * target-tiled kernel instead of the row-major one. */
+	SPICE_FLAG_FLUSH_TILED = 1u << 1
 };
 
 /* Attribute type codes reported by spice_model_info. */

@@ -10,7 +10,9 @@
 //   acc[a]   receive accumulator of attribute a   fp32 / int64 (fixed point) / int32
 //   sa[a]    attribute a of every synapse         CSR order (plastic models only)
 //   ring     int32[(D+1) * N], counts uint32[D+1] spike ids of the last D+1 steps
-//   hist     uint64[N]   bit b = spiked b steps ago     (plastic models)
+//   hring    uint64[HWORDS * N]  spike-history ring: bit (t mod H) of word (t mod H)/64 of
+//                        neuron i set iff i spiked at step t (H = 256 steps, plastic models)
+//   last_spk int32[N]    latest spike step
 //   synced   int32[N]    row s is up to date through this step
 //   pend     int32[N]    row s has an undelivered pre event at synced+1
 //   hasp     uint8[N]    row s has at least one plastic synapse
@@ -19,7 +21,7 @@
 //                        (synced, pend) at that time; counters double-buffered by step parity
 //   pivots   int32[N * (ntiles+1)]  per row and target tile: offset of the tile's first target
 //   xtr/ytr  f32[N]      neuron-local pre/post traces (trace synapses)
-//   xh/yh    f32[64 * N] their values over the last 64 steps (ring, slot = step mod 64)
+//   xh/yh    f32[H * N]  their values over the last H steps (ring, slot = step mod H)
 #include <cstdint>
 
 namespace spice::hipx
@@ -70,7 +72,8 @@ struct dview
 	std::int32_t * ring;
 	std::uint32_t * counts;
 
-	unsigned long long * hist;
+	unsigned long long * hring; // [HWORDS][N] spike-history ring
+	std::int32_t * last_spk;    // [N] latest spike step (-1: none)
 	std::int32_t * synced;
 	std::int32_t * pend;
 	std::uint8_t * hasp;

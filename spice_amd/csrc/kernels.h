@@ -32,8 +32,9 @@
 
 namespace spice::hipx
 {
-constexpr int HISTORY = 64; // post-spike history window (steps) of lazy plasticity
-constexpr int FLUSH_AGE = HISTORY - 1;
+constexpr int HISTORY = 256;            // spike-history window (steps) of lazy plasticity
+constexpr int HWORDS = HISTORY / 64;     // 64-bit words per neuron in the history ring
+constexpr int FLUSH_AGE = HISTORY - 1;   // rows are brought up to date at least this often
 
 template <typename T>
 __device__ inline T ld( void const * base, std::int64_t i )
@@ -163,7 +164,7 @@ __device__ inline float step_dt( dview const & v ) { return v.DT - v.ctl->c; }
 template <int K>
 __device__ inline void block_add_stats( dview const & v, unsigned long long const ( &x )[K], int const ( &idx )[K] )
 {
-	__shared__ unsigned long long part[4][K]; // blockDim.x <= 256: at most 4 waves
+	__shared__ unsigned long long part[16][K]; // blockDim.x <= 1024: at most 16 waves
 	int const w = threadIdx.x >> 6;
 #pragma unroll
 	for( int k = 0; k < K; k++ )
@@ -204,80 +205,102 @@ __device__ inline void store_tuple( typename Decl::tuple_t const & s, void * con
 }
 
 // ---- lazy plasticity ---------------------------------------------------------------------
-// Applies, in time order, the plasticity of every step in (synced, now] to one synapse:
-// the pending pre event at synced+1 (if pend) and the target's post spikes, given as
-// history bits (bit b <=> target spiked at step now - b). Event-free stretches are merged
-// into the next call's nsteps (the contract's n-step form), and a trailing event-free
-// stretch is applied with pre = post = false. Equivalent to the CPU checker's dense
-// per-step updates (reference counterpart: algorithm.cu:283-294).
+// Spike history: bit (t mod HISTORY) of ring word ((t mod HISTORY) / 64) of neuron j is set
+// iff j spiked at step t, valid for the last HISTORY steps; last_spk[j] is j's latest spike.
+
+__device__ inline bool spiked_at( dview const & v, std::int32_t j, std::int32_t e )
+{
+	std::int64_t const w = ( e & ( HISTORY - 1 ) ) >> 6;
+	return ( v.hring[w * v.N + j] >> ( e & 63 ) ) & 1ull;
+}
+
+// f(e) for every step e in [first, last] at which neuron j spiked, in increasing order
+// (last - first < HISTORY). All history words of j are loaded up front (independent loads).
+template <typename F>
+__device__ inline void for_each_spike( dview const & v, std::int32_t j, std::int32_t first, std::int32_t last, F && f )
+{
+	if( first > last ) return;
+	unsigned long long words[HWORDS];
+#pragma unroll
+	for( int w = 0; w < HWORDS; w++ ) words[w] = v.hring[std::int64_t( w ) * v.N + j];
+	for( std::int32_t e = first; e <= last; )
+	{
+		int const idx = e & ( HISTORY - 1 );
+		int const b0 = idx & 63;
+		int const cnt = ( 64 - b0 ) < ( last - e + 1 ) ? ( 64 - b0 ) : ( last - e + 1 );
+		unsigned long long word = 0;
+#pragma unroll
+		for( int w = 0; w < HWORDS; w++ )
+			if( w == ( idx >> 6 ) ) word = words[w];
+		unsigned long long bits = word >> b0;
+		if( cnt < 64 ) bits &= ( 1ull << cnt ) - 1ull;
+		while( bits )
+		{
+			int const k = __builtin_ctzll( bits );
+			f( e + k );
+			bits &= bits - 1ull;
+		}
+		e += cnt;
+	}
+}
+
+// Applies, in time order, the plasticity of every step in (synced, now] to one synapse
+// src -> j: the pending pre event at synced+1 (if pend) and j's post spikes. Event-free
+// stretches are merged into the next call's nsteps (the contract's n-step form) and a
+// trailing event-free stretch is applied with pre = post = false. Equivalent to the CPU
+// checker's dense per-step updates (reference counterpart: algorithm.cu:283-294).
 template <typename Syn, typename Iter, typename Bak>
 __device__ inline void replay(
-    Iter syn, unsigned long long hist, std::int32_t now, std::int32_t synced, std::int32_t pend, float dt, snn_info info, Bak & bak )
+    Iter syn, dview const & v, std::int32_t j, std::int32_t now, std::int32_t synced, std::int32_t pend, float dt, snn_info info, Bak & bak )
 {
-	std::int32_t const n = now - synced;
-	if( n <= 0 ) return;
-	unsigned long long h = n >= 64 ? hist : ( hist & ( ( 1ull << n ) - 1ull ) );
-	std::int32_t t = synced;
+	if( now <= synced ) return;
+	std::int32_t t = synced, first = synced + 1;
+	std::int32_t const ls = v.last_spk[j];
 	if( pend )
 	{
-		int const b = n - 1;
-		bool const post = ( h >> b ) & 1ull;
-		h &= ~( 1ull << b );
+		bool const post = ls >= first && spiked_at( v, j, first );
 		Syn::update( syn, 1, true, post, dt, info, bak );
-		t = synced + 1;
+		t = first++;
 	}
-	while( h )
-	{
-		int const b = 63 - __clzll( h );
-		std::int32_t const e = now - b;
+	for_each_spike( v, j, first, ls < now ? ls : now, [&]( std::int32_t e ) {
 		Syn::update( syn, e - t, false, true, dt, info, bak );
 		t = e;
-		h &= ~( 1ull << b );
-	}
+	} );
 	if( t < now ) Syn::update( syn, now - t, false, false, dt, info, bak );
 }
 
-// Neuron-local-trace plasticity (trace_synapse models): same event sequence as replay(), but
-// an event calls Syn::on_event with the source's pre trace and the target's post trace at
-// the event's step, read from the 64-step trace rings. Event-free steps change nothing by
-// contract, so a synapse whose target did not spike in the window (and with no pending pre
-// event) is not touched at all. Returns whether the synapse changed.
-__device__ inline unsigned long long window_mask( std::int32_t n ) { return n >= 64 ? ~0ull : ( ( 1ull << n ) - 1ull ); }
-
+// Neuron-local-trace plasticity (trace_synapse models): the same event sequence, but an
+// event calls Syn::on_event with the source's pre trace and the target's post trace at the
+// event's step, read from the trace rings. Event-free steps change nothing by contract, so a
+// synapse whose target did not spike in the window (and with no pending pre event) is not
+// touched at all. Returns whether the synapse changed.
 template <typename Syn, typename Iter, typename Bak>
 __device__ inline bool replay_traced(
-    Iter syn,
-    unsigned long long hist,
-    std::int32_t now,
-    std::int32_t synced,
-    std::int32_t pend,
-    std::int32_t s,
-    std::int32_t j,
-    dview const & v,
-    snn_info info,
-    Bak & bak )
+    Iter syn, dview const & v, std::int32_t s, std::int32_t j, std::int32_t now, std::int32_t synced, std::int32_t pend, snn_info info, Bak & bak )
 {
-	std::int32_t const n = now - synced;
-	if( n <= 0 ) return false;
-	unsigned long long h = hist & window_mask( n );
-	if( !pend && !h ) return false;
+	if( now <= synced ) return false;
+	std::int32_t const ls = v.last_spk[j];
+	if( !pend && ls <= synced ) return false;
 	std::int64_t const N = v.N;
+	std::int32_t first = synced + 1;
 	if( pend )
 	{
-		int const b = n - 1;
-		bool const post = ( h >> b ) & 1ull;
-		h &= ~( 1ull << b );
-		std::int64_t const slot = ( synced + 1 ) & ( HISTORY - 1 );
+		bool const post = ls >= first && spiked_at( v, j, first );
+		std::int64_t const slot = first & ( HISTORY - 1 );
 		Syn::on_event( syn, true, post, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
+		first++;
 	}
-	while( h )
-	{
-		int const b = 63 - __clzll( h );
-		std::int64_t const slot = ( now - b ) & ( HISTORY - 1 );
+	for_each_spike( v, j, first, ls < now ? ls : now, [&]( std::int32_t e ) {
+		std::int64_t const slot = e & ( HISTORY - 1 );
 		Syn::on_event( syn, false, true, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
-		h &= ~( 1ull << b );
-	}
+	} );
 	return true;
+}
+
+// Whether synapse src -> j has an event in (synced, now] (trace synapses only need those).
+__device__ inline bool has_event( dview const & v, std::int32_t j, std::int32_t synced, std::int32_t pend )
+{
+	return pend || v.last_spk[j] > synced;
 }
 
 // ---- kernels -------------------------------------------------------------------------------
@@ -328,111 +351,10 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 	}
 }
 
-template <typename Model, bool FIXED>
-__global__ __launch_bounds__( 256 ) void k_receive( dview v )
-{
-	using N = typename Model::neuron;
-	using S = typename Model::synapse;
-	constexpr bool plastic = is_plastic_v<Model>;
-
-	std::int64_t const T = v.ctl->step;
-	int const D1 = v.D + 1;
-	int const slot = int( ( T + 1 ) % D1 ); // ring entry of step T - D
-	std::int64_t const nitems = T < v.D ? 0 : std::int64_t( v.counts[slot] ) * v.parts;
-	std::int32_t const * spk = v.ring + std::int64_t( slot ) * v.N;
-	float const dt = step_dt( v );
-	snn_info const info = info_of( v );
-	int const lane = lane_id();
-	std::int64_t const wave0 = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
-	std::int64_t const nwaves = std::int64_t( gridDim.x ) * ( blockDim.x >> 6 );
-
-	unsigned long long ev = 0, pev = 0;
-	for( std::int64_t it = wave0; it < nitems; it += nwaves )
-	{
-		std::int64_t const k = it / v.parts;
-		std::int64_t const p = it - k * v.parts;
-		std::int32_t const s = uniform( spk[k] );
-		std::int64_t const lo = v.row_ptr[s], hi = v.row_ptr[s + 1];
-		std::int64_t const chunk = ( ( hi - lo + v.parts - 1 ) / v.parts + 255 ) & ~std::int64_t( 255 );
-		std::int64_t const beg = lo + p * chunk;
-		std::int64_t const end = beg + chunk < hi ? beg + chunk : hi;
-		if( beg >= end ) continue;
-		ev += std::uint64_t( end - beg );
-
-		constexpr int U = 4; // slots per lane per iteration: independent loads in flight
-		[[maybe_unused]] std::int32_t synced = 0, pend = 0;
-		if constexpr( plastic )
-		{
-			synced = v.synced[s];
-			pend = v.pend[s];
-		}
-		for( std::int64_t base = beg; base < end; base += 64 * U )
-		{
-			std::int32_t j[U];
-			bool ok[U];
-#pragma unroll
-			for( int u = 0; u < U; u++ )
-			{
-				std::int64_t const e = base + lane + 64 * u;
-				ok[u] = e < end;
-				j[u] = ok[u] ? v.col[e] : 0;
-			}
-			if constexpr( plastic )
-			{
-				typename S::tuple_t sy[U];
-				bool pl[U];
-				unsigned long long hj[U];
-#pragma unroll
-				for( int u = 0; u < U; u++ )
-				{
-					if( ok[u] ) load_tuple<S>( sy[u], v.sa, base + lane + 64 * u );
-					pl[u] = ok[u] && S::plastic( s, j[u], info );
-					hj[u] = pl[u] ? v.hist[j[u]] : 0ull;
-				}
-#pragma unroll
-				for( int u = 0; u < U; u++ )
-				{
-					if( !ok[u] ) continue;
-					std::int64_t const e = base + lane + 64 * u;
-					auto bak = make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE );
-					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e ) };
-					if( pl[u] )
-					{
-						bool changed;
-						if constexpr( has_neuron_traces_v<S> )
-							changed = replay_traced<S>( itr, hj[u], std::int32_t( T - 1 ), synced, pend, s, j[u], v, info, bak );
-						else
-						{
-							replay<S>( itr, hj[u], std::int32_t( T - 1 ), synced, pend, dt, info, bak );
-							changed = true;
-						}
-						if( changed )
-						{
-							store_tuple<S>( sy[u], v.sa, e );
-							pev++;
-						}
-					}
-					N::receive( s, recv_iter<N>{ &v, j[u] }, itr, info, bak );
-				}
-			}
-			else
-			{
-#pragma unroll
-				for( int u = 0; u < U; u++ )
-				{
-					if( !ok[u] ) continue;
-					auto bak = make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE );
-					N::receive( s, recv_iter<N>{ &v, j[u] }, no_syn_iter{}, info, bak );
-				}
-			}
-		}
-	}
-	block_add_stats<2>( v, { lane == 0 ? ev : 0ull, pev }, { ST_EVENTS, ST_PLASTIC } );
-}
-
 // ---- target-tiled spike delivery -----------------------------------------------------------
 constexpr int TILE_MAX = 1024; // targets per tile (LDS accumulators)
-constexpr int SPK_CHUNK = 256; // delivered spikes staged in LDS per pass
+constexpr int TBS = 1024;      // threads per tile workgroup (16 waves)
+constexpr int SPK_CHUNK = TBS; // rows (delivered spikes / flushed rows) staged in LDS per pass
 
 // Per row r and tile t: offset (within the row) of the first target >= t * tile_w, for
 // t = 0 .. ntiles (the last entry is the row length). One wave per row, binary searches.
@@ -507,6 +429,36 @@ struct tile_backend : dev_backend<FIXED>
 	}
 };
 
+// Exclusive scan of one int32 per thread over a TBS-thread block: returns this thread's
+// exclusive prefix and writes the block total to *total (LDS). Contains barriers.
+__device__ inline std::int32_t block_exclusive_scan( std::int32_t x, std::int32_t * wsum, std::int32_t * total )
+{
+	std::uint64_t const incl = wave_inclusive_scan( std::uint64_t( std::uint32_t( x ) ) );
+	int const tid = threadIdx.x;
+	if( lane_id() == 63 ) wsum[tid >> 6] = std::int32_t( incl );
+	__syncthreads();
+	std::int32_t wb = 0;
+	for( int w = 0; w < ( tid >> 6 ); w++ ) wb += wsum[w];
+	if( tid == TBS - 1 ) *total = wb + std::int32_t( incl );
+	__syncthreads();
+	return wb + std::int32_t( incl ) - x;
+}
+
+// Largest k in [0, cnt) with pre[k] <= f (pre non-decreasing, pre[0] = 0).
+__device__ inline std::int32_t segment_of( std::int32_t const * pre, std::int32_t cnt, std::int32_t f )
+{
+	std::int32_t a = 0, b = cnt - 1;
+	while( a < b )
+	{
+		std::int32_t const m = ( a + b + 1 ) >> 1;
+		if( pre[m] <= f )
+			a = m;
+		else
+			b = m - 1;
+	}
+	return a;
+}
+
 // Delivers the spikes of step T - D, one workgroup per tile of tile_w target neurons:
 // the workgroup stages a chunk of spikes (id, segment of the row inside the tile, and for
 // plastic models the row's lazy state) in LDS, forms the segments' prefix sums, and walks
@@ -515,14 +467,14 @@ struct tile_backend : dev_backend<FIXED>
 // atomics; at the end the tile's sums go to the global accumulators with plain stores:
 // one workgroup owns each tile, so no global atomics are needed.
 template <typename Model, bool FIXED>
-__global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
+__global__ __launch_bounds__( TBS ) void k_receive_tiled( dview v )
 {
 	using N = typename Model::neuron;
 	using S = typename Model::synapse;
 	constexpr bool plastic = is_plastic_v<Model>;
 	constexpr std::uint32_t rmask = receive_mask_v<N>;
 	constexpr int nacc = __builtin_popcount( rmask ) > 0 ? __builtin_popcount( rmask ) : 1;
-	constexpr int U = 4;
+	constexpr int U = 2;
 
 	__shared__ unsigned long long acc[nacc * TILE_MAX];
 	__shared__ std::int64_t seg_beg[SPK_CHUNK];
@@ -530,7 +482,7 @@ __global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
 	__shared__ std::int32_t seg_src[SPK_CHUNK];
 	__shared__ std::int32_t seg_synced[plastic ? SPK_CHUNK : 1];
 	__shared__ std::int32_t seg_pend[plastic ? SPK_CHUNK : 1];
-	__shared__ std::int32_t wsum[4];
+	__shared__ std::int32_t wsum[TBS / 64];
 
 	std::int64_t const T = v.ctl->step;
 	int const D1 = v.D + 1;
@@ -546,7 +498,7 @@ __global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
 	{
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
-		for( int x = tid; x < nacc * TILE_MAX; x += 256 ) acc[x] = 0ull;
+		for( int x = tid; x < nacc * TILE_MAX; x += TBS ) acc[x] = 0ull;
 		__syncthreads();
 
 		for( std::int32_t c0 = 0; c0 < nspk; c0 += SPK_CHUNK )
@@ -568,17 +520,12 @@ __global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
 				}
 			}
 			// exclusive block scan of the segment lengths
-			std::uint64_t const incl = wave_inclusive_scan( std::uint64_t( len ) );
-			if( lane_id() == 63 ) wsum[tid >> 6] = std::int32_t( incl );
+			std::int32_t const pre = block_exclusive_scan( len, wsum, &seg_pre[SPK_CHUNK] );
+			if( tid < cnt ) seg_pre[tid] = pre;
 			__syncthreads();
-			std::int32_t wb = 0;
-			for( int w = 0; w < ( tid >> 6 ); w++ ) wb += wsum[w];
-			if( tid < cnt ) seg_pre[tid] = wb + std::int32_t( incl ) - len;
-			if( tid == 255 ) seg_pre[cnt] = wb + std::int32_t( incl );
-			__syncthreads();
-			std::int32_t const total = seg_pre[cnt];
+			std::int32_t const total = seg_pre[SPK_CHUNK];
 
-			for( std::int32_t f0 = 0; f0 < total; f0 += 256 * U )
+			for( std::int32_t f0 = 0; f0 < total; f0 += TBS * U )
 			{
 				std::int32_t kk[U];
 				std::int64_t e[U];
@@ -587,17 +534,9 @@ __global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
 #pragma unroll
 				for( int u = 0; u < U; u++ )
 				{
-					std::int32_t const f = f0 + tid + 256 * u;
+					std::int32_t const f = f0 + tid + TBS * u;
 					ok[u] = f < total;
-					std::int32_t a = 0, b = cnt - 1; // largest k with seg_pre[k] <= f
-					while( a < b )
-					{
-						std::int32_t const m = ( a + b + 1 ) >> 1;
-						if( seg_pre[m] <= f )
-							a = m;
-						else
-							b = m - 1;
-					}
+					std::int32_t const a = segment_of( seg_pre, cnt, f );
 					kk[u] = a;
 					e[u] = seg_beg[a] + ( f - seg_pre[a] );
 					j[u] = ok[u] ? v.col[e[u]] : 0;
@@ -619,10 +558,10 @@ __global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
 							std::int32_t const synced = seg_synced[kk[u]], pend = seg_pend[kk[u]];
 							bool changed;
 							if constexpr( has_neuron_traces_v<S> )
-								changed = replay_traced<S>( itr, v.hist[j[u]], std::int32_t( T - 1 ), synced, pend, s, j[u], v, info, bak );
+								changed = replay_traced<S>( itr, v, s, j[u], std::int32_t( T - 1 ), synced, pend, info, bak );
 							else
 							{
-								replay<S>( itr, v.hist[j[u]], std::int32_t( T - 1 ), synced, pend, dt, info, bak );
+								replay<S>( itr, v, j[u], std::int32_t( T - 1 ), synced, pend, dt, info, bak );
 								changed = true;
 							}
 							if( changed )
@@ -649,7 +588,7 @@ __global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
 				using A = std::conditional_t<std::is_floating_point_v<T>, std::conditional_t<FIXED, long long, float>, T>;
 				constexpr int a = __builtin_popcount( rmask & ( ( 1u << I ) - 1u ) );
 				A const * src = reinterpret_cast<A const *>( acc + a * TILE_MAX );
-				for( int x = tid; x < tw; x += 256 )
+				for( int x = tid; x < tw; x += TBS )
 				{
 					A const val = src[x];
 					if( val != A( 0 ) ) st<A>( v.acc[I], t0 + x, val );
@@ -659,6 +598,99 @@ __global__ __launch_bounds__( 256 ) void k_receive_tiled( dview v )
 		__syncthreads();
 	}
 	block_add_stats<2>( v, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
+}
+
+// Brings the rows queued by this step's neuron update up to date (lazy plasticity), one
+// workgroup per target tile: the queued rows' plastic segments inside the tile are walked as
+// one flat index space, and only synapses with an event in their row's window are read and
+// written.
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
+{
+	using S = typename Model::synapse;
+	constexpr int U = 2;
+	__shared__ std::int64_t seg_beg[SPK_CHUNK];
+	__shared__ std::int32_t seg_pre[SPK_CHUNK + 1];
+	__shared__ std::int32_t seg_src[SPK_CHUNK];
+	__shared__ std::int32_t seg_synced[SPK_CHUNK];
+	__shared__ std::int32_t seg_pend[SPK_CHUNK];
+	__shared__ std::int32_t wsum[TBS / 64];
+
+	std::int64_t const T = v.ctl->step;
+	std::int32_t const nrows = std::int32_t( v.fl_count[T & 1] );
+	float const dt = step_dt( v );
+	snn_info const info = info_of( v );
+	int const tid = threadIdx.x;
+	unsigned long long touched = 0, slots = 0;
+	if( nrows > 0 )
+		for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x )
+		{
+			std::int32_t const t0 = tile * v.tile_w;
+				for( std::int32_t c0 = 0; c0 < nrows; c0 += SPK_CHUNK )
+			{
+				std::int32_t const cnt = ( nrows - c0 ) < SPK_CHUNK ? ( nrows - c0 ) : SPK_CHUNK;
+				std::int32_t len = 0;
+				if( tid < cnt )
+				{
+					std::int32_t const r = v.fl_id[c0 + tid];
+					std::int32_t const * pv = v.pivots + std::int64_t( r ) * ( v.ntiles + 1 ) + tile;
+					std::int32_t a = pv[0], b = pv[1];
+					std::int32_t const pa = v.pl_lo[r], pb = v.pl_hi[r];
+					a = a > pa ? a : pa;
+					b = b < pb ? b : pb;
+					len = b > a ? b - a : 0;
+					seg_beg[tid] = v.row_ptr[r] + a;
+					seg_src[tid] = r;
+					seg_synced[tid] = v.fl_synced[c0 + tid];
+					seg_pend[tid] = v.fl_pend[c0 + tid];
+				}
+				std::int32_t const pre = block_exclusive_scan( len, wsum, &seg_pre[SPK_CHUNK] );
+				if( tid < cnt ) seg_pre[tid] = pre;
+				__syncthreads();
+				std::int32_t const total = seg_pre[SPK_CHUNK];
+				for( std::int32_t f0 = 0; f0 < total; f0 += TBS * U )
+				{
+					std::int32_t kk[U];
+					std::int64_t e[U];
+					std::int32_t j[U];
+					bool ok[U];
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						std::int32_t const f = f0 + tid + TBS * u;
+						ok[u] = f < total;
+						kk[u] = segment_of( seg_pre, cnt, f );
+						e[u] = seg_beg[kk[u]] + ( f - seg_pre[kk[u]] );
+						j[u] = ok[u] ? v.col[e[u]] : 0;
+					}
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						if( !ok[u] ) continue;
+						std::int32_t const s = seg_src[kk[u]];
+						if( !S::plastic( s, j[u], info ) ) continue;
+						std::int32_t const synced = seg_synced[kk[u]], pend = seg_pend[kk[u]];
+						if constexpr( has_neuron_traces_v<S> )
+							if( !has_event( v, j[u], synced, pend ) ) continue;
+						typename S::tuple_t sy;
+						load_tuple<S>( sy, v.sa, e[u] );
+						auto bak = make_backend<FIXED>(
+						    v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
+						reg_iter<typename S::tuple_t> itr{ sy, std::uint32_t( e[u] ) };
+						if constexpr( has_neuron_traces_v<S> )
+							replay_traced<S>( itr, v, s, j[u], std::int32_t( T ), synced, pend, info, bak );
+						else
+							replay<S>( itr, v, j[u], std::int32_t( T ), synced, pend, dt, info, bak );
+						store_tuple<S>( sy, v.sa, e[u] );
+						touched++;
+					}
+				}
+				if( tid == 0 ) slots += std::uint64_t( total );
+				__syncthreads();
+			}
+		}
+	block_add_stats<3>(
+	    v, { touched, slots, blockIdx.x == 0 && tid == 0 ? std::uint64_t( nrows ) : 0ull }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
 template <typename Model, bool FIXED>
@@ -720,17 +752,24 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 		if constexpr( plastic )
 		{
 			using S = typename Model::synapse;
-			unsigned long long h = v.hist[i];
+			std::int64_t const N64 = v.N;
 			std::int32_t synced = v.synced[i], pend = v.pend[i];
 			bool dirty = false;
-			bool const arrived = ( h >> ( v.D - 1 ) ) & 1ull; // this neuron's spike reached its synapses now
+			// this neuron's spike of step T - D reaches its synapses now
+			bool const arrived = T >= v.D && spiked_at( v, std::int32_t( i ), std::int32_t( T - v.D ) );
 			if( arrived )
 			{
 				synced = std::int32_t( T - 1 );
 				pend = 1;
 				dirty = true;
 			}
-			v.hist[i] = ( h << 1 ) | ( spiked ? 1ull : 0ull );
+			{ // record this step in the history ring
+				std::int64_t const w = ( T & ( HISTORY - 1 ) ) >> 6;
+				unsigned long long const bit = 1ull << ( T & 63 );
+				unsigned long long const word = v.hring[w * N64 + i];
+				v.hring[w * N64 + i] = spiked ? ( word | bit ) : ( word & ~bit );
+				if( spiked ) v.last_spk[i] = std::int32_t( T );
+			}
 			if constexpr( has_neuron_traces_v<S> )
 			{
 				float const d = S::trace_decay( dt );
@@ -739,8 +778,8 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 				v.xtr[i] = x;
 				v.ytr[i] = y;
 				std::int64_t const slot = T & ( HISTORY - 1 );
-				v.xh[slot * v.N + i] = x;
-				v.yh[slot * v.N + i] = y;
+				v.xh[slot * N64 + i] = x;
+				v.yh[slot * N64 + i] = y;
 			}
 			// Rows are flushed before their window overflows; a row never processed yet gets its
 			// first flush at step (i mod FLUSH_AGE), which spreads flushes evenly over steps.
@@ -828,7 +867,6 @@ __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 		std::int64_t const chunk = ( ( hi - lo + v.fparts - 1 ) / v.fparts + 255 ) & ~std::int64_t( 255 );
 		std::int64_t const beg = lo + p * chunk;
 		std::int64_t const end = beg + chunk < hi ? beg + chunk : hi;
-		unsigned long long const wmask = window_mask( std::int32_t( T ) - synced );
 		for( std::int64_t base = beg; base < end; base += 64 * U )
 		{
 			std::int32_t j[U];
@@ -842,15 +880,15 @@ __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 			}
 #pragma unroll
 			for( int u = 0; u < U; u++ ) ok[u] = ok[u] && S::plastic( s, j[u], info );
-			unsigned long long hj[U];
+			std::int32_t ls[U];
 #pragma unroll
-			for( int u = 0; u < U; u++ ) hj[u] = ok[u] ? v.hist[j[u]] : 0ull;
+			for( int u = 0; u < U; u++ ) ls[u] = ok[u] ? v.last_spk[j[u]] : -1;
 #pragma unroll
 			for( int u = 0; u < U; u++ )
 			{
 				if( !ok[u] ) continue;
 				if constexpr( has_neuron_traces_v<S> )
-					if( !pend && !( hj[u] & wmask ) ) continue; // no event for this synapse in the window
+					if( !pend && ls[u] <= synced ) continue; // no event for this synapse in the window
 				std::int64_t const e = base + lane + 64 * u;
 				typename S::tuple_t sy;
 				load_tuple<S>( sy, v.sa, e );
@@ -858,9 +896,9 @@ __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 				    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
 				reg_iter<typename S::tuple_t> itr{ sy, std::uint32_t( e ) };
 				if constexpr( has_neuron_traces_v<S> )
-					replay_traced<S>( itr, hj[u], std::int32_t( T ), synced, pend, s, j[u], v, info, bak );
+					replay_traced<S>( itr, v, s, j[u], std::int32_t( T ), synced, pend, info, bak );
 				else
-					replay<S>( itr, hj[u], std::int32_t( T ), synced, pend, dt, info, bak );
+					replay<S>( itr, v, j[u], std::int32_t( T ), synced, pend, dt, info, bak );
 				store_tuple<S>( sy, v.sa, e );
 				touched++;
 			}
@@ -895,11 +933,11 @@ __global__ __launch_bounds__( 256 ) void k_sync_all( dview v, std::int32_t now, 
 		reg_iter<typename S::tuple_t> it{ sy, std::uint32_t( e ) };
 		if constexpr( has_neuron_traces_v<S> )
 		{
-			if( replay_traced<S>( it, v.hist[j], now, synced, pend, std::int32_t( r ), j, v, info, bak ) ) store_tuple<S>( sy, v.sa, e );
+			if( replay_traced<S>( it, v, std::int32_t( r ), j, now, synced, pend, info, bak ) ) store_tuple<S>( sy, v.sa, e );
 		}
 		else
 		{
-			replay<S>( it, v.hist[j], now, synced, pend, dt, info, bak );
+			replay<S>( it, v, j, now, synced, pend, dt, info, bak );
 			store_tuple<S>( sy, v.sa, e );
 		}
 	}

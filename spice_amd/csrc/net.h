@@ -87,8 +87,8 @@ public:
 	{
 		if( !( p.dt > 0.0f ) ) throw std::invalid_argument( "dt must be positive" );
 		if( p.delay < 1 ) throw std::invalid_argument( "delay must be >= 1" );
-		if( plastic && p.delay > HISTORY )
-			throw std::invalid_argument( "plastic models support delays of at most 64 steps" );
+		if( plastic && p.delay > HISTORY - 1 )
+			throw std::invalid_argument( "plastic models support delays of at most 255 steps" );
 		if( _g.n < 1 || _g.n > INT_MAX - 1024 ) throw std::invalid_argument( "invalid neuron count" );
 		static_assert( N::size <= MAXA && S::size <= MAXA, "at most 16 neuron/synapse attributes" );
 		allocate();
@@ -232,7 +232,8 @@ public:
 		SPICE_HIP( hipMemset( _counts.data(), 0, _counts.bytes() ) );
 		if constexpr( plastic )
 		{
-			SPICE_HIP( hipMemset( _hist.data(), 0, _hist.bytes() ) );
+			SPICE_HIP( hipMemset( _hring.data(), 0, _hring.bytes() ) );
+			SPICE_HIP( hipMemset( _last_spk.data(), 0xff, _last_spk.bytes() ) );
 			std::vector<std::int32_t> synced( _g.n, std::int32_t( step - 1 ) );
 			SPICE_HIP( hipMemcpy( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice ) );
 			SPICE_HIP( hipMemset( _pend.data(), 0, _pend.bytes() ) );
@@ -350,7 +351,8 @@ private:
 	dbuf<char> _na[MAXA], _acc[MAXA], _sa[MAXA];
 	dbuf<std::int32_t> _ring;
 	dbuf<std::uint32_t> _counts;
-	dbuf<unsigned long long> _hist;
+	dbuf<unsigned long long> _hring;
+	dbuf<std::int32_t> _last_spk;
 	dbuf<std::int32_t> _synced, _pend, _fl_id, _fl_synced, _fl_pend;
 	dbuf<std::uint8_t> _hasp;
 	dbuf<std::int32_t> _pl_lo, _pl_hi;
@@ -405,7 +407,8 @@ private:
 		SPICE_HIP( hipMemsetAsync( _counts.data(), 0, _counts.bytes(), _s ) );
 		if constexpr( plastic )
 		{
-			_hist.resize( n );
+			_hring.resize( std::size_t( HWORDS ) * n );
+			_last_spk.resize( n );
 			_synced.resize( n );
 			_pend.resize( n );
 			_hasp.resize( n );
@@ -415,7 +418,8 @@ private:
 			_fl_synced.resize( n );
 			_fl_pend.resize( n );
 			_fl_count.resize( 2 );
-			SPICE_HIP( hipMemsetAsync( _hist.data(), 0, _hist.bytes(), _s ) );
+			SPICE_HIP( hipMemsetAsync( _hring.data(), 0, _hring.bytes(), _s ) );
+			SPICE_HIP( hipMemsetAsync( _last_spk.data(), 0xff, _last_spk.bytes(), _s ) ); // -1
 			SPICE_HIP( hipMemsetAsync( _pend.data(), 0, _pend.bytes(), _s ) );
 			SPICE_HIP( hipMemsetAsync( _fl_count.data(), 0, _fl_count.bytes(), _s ) );
 			if constexpr( traced )
@@ -451,7 +455,8 @@ private:
 		}
 		v.ring = _ring.data();
 		v.counts = _counts.data();
-		v.hist = _hist.data();
+		v.hring = _hring.data();
+		v.last_spk = _last_spk.data();
 		v.synced = _synced.data();
 		v.pend = _pend.data();
 		v.hasp = _hasp.data();
@@ -472,7 +477,7 @@ private:
 		// occupies several waves; grids are persistent (work is read from device memory).
 		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
 		v.parts = std::max( 1, int( mean_row / 1024.0 + 0.5 ) );
-		v.fparts = std::max( 1, int( mean_row / 2048.0 + 0.5 ) );
+		v.fparts = std::max( 1, int( mean_row / 256.0 + 0.5 ) );
 		int const cus = device_cu_count();
 		_grid_update = int( ( n + 255 ) / 256 );
 		// persistent grids: up to 8 waves per SIMD (the kernels use few VGPRs)
@@ -517,20 +522,32 @@ private:
 	template <bool F>
 	void launch_step_t( dview const & v )
 	{
-		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, 256, 0, _s>>>( v ) );
+		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
-		if constexpr( plastic ) SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( v ) );
+		if constexpr( plastic )
+		{
+			if( _p.flags & SPICE_FLAG_FLUSH_TILED )
+				SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
+			else
+				SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( v ) );
+		}
 	}
 
 	template <bool F>
 	void launch_step_events( hipEvent_t * e, dview const & v )
 	{
 		SPICE_HIP( hipEventRecord( e[0], _s ) );
-		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, 256, 0, _s>>>( v ) );
+		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[1], _s ) );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[2], _s ) );
-		if constexpr( plastic ) SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( v ) );
+		if constexpr( plastic )
+		{
+			if( _p.flags & SPICE_FLAG_FLUSH_TILED )
+				SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
+			else
+				SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( v ) );
+		}
 		SPICE_HIP( hipEventRecord( e[3], _s ) );
 	}
 

@@ -71,7 +71,7 @@ class Layout:
 class Snn:
     """One network simulated on the GPU (cuda::snn<Model> counterpart)."""
 
-    def __init__(self, model, layout, dt, delay=1, seed=1337, fixed_point=False, _handle=None):
+    def __init__(self, model, layout, dt, delay=1, seed=1337, fixed_point=False, _handle=None, flags=0):
         self.model = model
         self._mid = model_id(model)
         self._h = _c.c_void_p()
@@ -83,7 +83,7 @@ class Snn:
                 layout = Layout(*layout) if isinstance(layout, tuple) else Layout(layout)
             pops_a, conns_a = layout._c_args()
             _lib.check(L.spice_hip_create(self._mid, _u64p(pops_a), len(layout.pops), conns_a, len(layout.conns),
-                                          float(dt), int(delay), int(seed), self._flags(fixed_point),
+                                          float(dt), int(delay), int(seed), self._flags(fixed_point) | int(flags),
                                           _c.byref(self._h)))
         self._nattrs = neuron_attrs(model)
         self._sattrs = synapse_attrs(model)

@@ -124,12 +124,12 @@ def test_lazy_stdp_generic_contract_matches_dense(gpu):
         np.testing.assert_allclose(sd[k], sh[k], rtol=2e-5, atol=1e-9, err_msg=k)
 
 
-@pytest.mark.parametrize("delay", [1, 15, 64])
+@pytest.mark.parametrize("delay", [1, 15, 255])
 def test_lazy_stdp_neuron_traces_bit_exact(gpu, delay):
     """Trace-synapse plasticity (brunel_3pop_stdp): event-only lazy replay with neuron-local
     traces equals the checker's dense per-step plasticity bit for bit (fixed-point mode):
     spikes every step, neuron state and every weight, including flushes of rows older than the
-    64-step window (700 steps) and the largest supported delay."""
+    256-step history window (700 steps) and the largest supported delay (255)."""
     model = "brunel_3pop_stdp"
     lay = three_pop_layout(1500)
     h, d, _ = _run_pair(model, lay, delay, 700, True, check_every=100)
@@ -161,3 +161,10 @@ def test_cpp_drop_in_api(gpu):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_plastic_delay_limit(gpu):
+    """Plastic models keep a 256-step spike history, so delays above 255 are rejected."""
+    with pytest.raises(ValueError):
+        Snn("brunel_3pop_stdp", Layout(*three_pop_layout(500)), 1e-4, 256)
+    Snn("brunel_3pop", Layout(*three_pop_layout(500)), 1e-4, 1000).close()  # static models: any delay
