@@ -1,6 +1,6 @@
 #pragma once
 // Three-population Brunel network: a 20 Hz Poisson input population (the first N/5
-// neurons), then excitatory (4/5 of the rest) and inhibitory LIF populations. LIF: fp32 V,
+// neurons; snn_info::param[0] > 0 sets another rate, e.g. for parameter sweeps), then excitatory (4/5 of the rest) and inhibitory LIF populations. LIF: fp32 V,
 // tau 20 ms, threshold 20 mV, reset to 10 mV, 20 refractory steps (2 ms at dt = 0.1 ms).
 // Fixed fp32 weights scaling as 1/N like the reference's (+5/N V from Poisson and excitatory
 // sources, -25/N V from inhibitory ones: the reference's 1 : -5 ratio), applied to
@@ -52,7 +52,7 @@ struct brunel_3pop : model
 
 			if( n.id() < static_cast<uint_>( num_poisson( info ) ) )
 			{
-				float const rate = 20; // Hz
+				float const rate = info.param[0] > 0.0f ? info.param[0] : 20.0f; // Hz (param[0] overrides)
 				return bak.rand() < ( rate * dt );
 			}
 			if( --get<Twait>( n ) <= 0 )

@@ -86,6 +86,22 @@ typedef struct
 	float p;
 } spice_edge;
 
+/* One network of a batch (or a single network with per-net model parameters): layout as
+ * (pops, conns), its delay, its seed and up to 4 model parameters (snn_info::param; 0 = the
+ * model's default, e.g. brunel_3pop's Poisson rate in Hz). */
+typedef struct
+{
+	const uint64_t * pops;
+	int32_t npops;
+	const spice_connection * conns;
+	int32_t nconns;
+	int32_t delay;
+	uint64_t seed;
+	float params[4];
+} spice_net_desc;
+
+typedef struct spice_batch spice_batch;
+
 /* Counters accumulated on the device since creation (or the last reset). */
 typedef struct
 {
@@ -147,6 +163,9 @@ int spice_hip_create(
     uint32_t flags,
     spice_net ** out_net );
 
+/* spice_hip_create from a descriptor (adds per-net model parameters). */
+int spice_hip_create_desc( int32_t model, const spice_net_desc * desc, float dt, uint32_t flags, spice_net ** out_net );
+
 /* Replaces cuda::snn<Model>::snn(std::vector<int> const& adj, size_t width, float dt, int delay)
  * (spice/cuda/snn.h:26-32, spice/cuda/snn.d.cpp:89-121): imports an ELL adjacency
  * (num_neurons x width, each row sorted, padded with INT_MAX). width must be a multiple of 32. */
@@ -204,6 +223,22 @@ int spice_hip_run_profiled( spice_net * net, int64_t nsteps, double * kernel_ms 
 
 /* Waits for all work of the net. */
 int spice_hip_synchronize( spice_net * net );
+
+/* ---- batches: many independent networks stepped together -------------------------------- */
+/* No counterpart in the reference (it steps one network per cuda::snn). A batch holds nnets
+ * networks of one static model (sizes, delays, seeds and parameters may differ; dt and flags
+ * are shared) and steps all of them with one launch per step phase (a captured HIP graph).
+ * Each network's results equal those of the same network stepped alone. */
+int spice_batch_create( int32_t model, int32_t nnets, const spice_net_desc * descs, float dt, uint32_t flags, spice_batch ** out_batch );
+int spice_batch_destroy( spice_batch * batch );
+/* Enqueues nsteps steps of every network in the batch (asynchronous). */
+int spice_batch_run( spice_batch * batch, int64_t nsteps );
+int spice_batch_synchronize( spice_batch * batch );
+int spice_batch_size( spice_batch * batch, int32_t * out_nnets );
+/* Borrowed handle of network i, valid while the batch lives: every query / export / spike
+ * readback of this header works on it (they wait for the batch first); stepping it alone,
+ * setting its clock or destroying it is rejected. */
+int spice_batch_net( spice_batch * batch, int32_t i, spice_net ** out_net );
 
 /* ---- queries ---------------------------------------------------------------------------- */
 

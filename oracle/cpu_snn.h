@@ -172,6 +172,7 @@ struct net_base
 	virtual void import_synapse_attr( int a, void const * in ) = 0;
 	virtual void get_clock( std::int64_t * i, float * sum, float * c ) const = 0;
 	virtual bool export_traces( float * x, float * y ) const = 0;
+	virtual void set_params( float const * p ) = 0;
 	virtual void set_clock( std::int64_t i, float sum, float c ) = 0;
 };
 
@@ -337,6 +338,11 @@ public:
 	void import_neuron_attr( int a, void const * in ) override { copy_in( _neurons, a, in ); }
 	void export_synapse_attr( int a, void * out ) const override { copy_out( _synapses, a, out ); }
 	void import_synapse_attr( int a, void const * in ) override { copy_in( _synapses, a, in ); }
+
+	void set_params( float const * p ) override
+	{
+		for( int k = 0; k < 4; k++ ) _info.param[k] = p[k];
+	}
 
 	bool export_traces( float * x, float * y ) const override
 	{

@@ -331,6 +331,12 @@ class CpuSnn:
             arr = np.ascontiguousarray(state[name], dt)
             _check(lib().oracle_import_synapse_attr(self._h, a, _c.c_void_p(arr.ctypes.data)))
 
+    def set_params(self, params):
+        """Per-network model parameters (snn_info::param), set before the first step."""
+        p = list(params or [])[:4]
+        p += [0.0] * (4 - len(p))
+        lib().oracle_set_params(self._h, (_c.c_float * 4)(*p))
+
     def traces(self):
         """Neuron-local (pre, post) traces of trace-synapse models."""
         n = self.num_neurons()

@@ -250,6 +250,11 @@ int oracle_import_synapse_attr( oracle::net_base * net, int32_t a, void const * 
 		net->import_synapse_attr( a, in );
 	} );
 }
+int oracle_set_params( oracle::net_base * net, float const * p )
+{
+	net->set_params( p );
+	return SPICE_OK;
+}
 int oracle_export_traces( oracle::net_base * net, float * x, float * y ) { return net->export_traces( x, y ) ? 0 : 1; }
 int oracle_get_clock( oracle::net_base * net, int64_t * i, float * sum, float * c )
 {

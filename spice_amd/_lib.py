@@ -36,6 +36,12 @@ class Edge(ctypes.Structure):
                 ("dst_first", ctypes.c_int32), ("dst_last", ctypes.c_int32), ("p", ctypes.c_float)]
 
 
+class NetDesc(ctypes.Structure):
+    _fields_ = [("pops", ctypes.POINTER(ctypes.c_uint64)), ("npops", ctypes.c_int32),
+                ("conns", ctypes.POINTER(Connection)), ("nconns", ctypes.c_int32), ("delay", ctypes.c_int32),
+                ("seed", ctypes.c_uint64), ("params", ctypes.c_float * 4)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
                 ("steps", "spikes", "events", "plastic_events", "flush_rows", "flush_synapses", "flush_slots")]
@@ -54,6 +60,13 @@ SIGNATURES = {
     "spice_model_info": (ctypes.c_int, [_I32, _PI32, _PI32, _PI32, _PI32]),
     "spice_hip_create": (ctypes.c_int, [_I32, _PU64, _I32, ctypes.POINTER(Connection), _I32, _F32, _I32,
                                         _U64, _U32, ctypes.POINTER(_P)]),
+    "spice_hip_create_desc": (ctypes.c_int, [_I32, ctypes.POINTER(NetDesc), _F32, _U32, ctypes.POINTER(_P)]),
+    "spice_batch_create": (ctypes.c_int, [_I32, _I32, ctypes.POINTER(NetDesc), _F32, _U32, ctypes.POINTER(_P)]),
+    "spice_batch_destroy": (ctypes.c_int, [_P]),
+    "spice_batch_run": (ctypes.c_int, [_P, _I64]),
+    "spice_batch_synchronize": (ctypes.c_int, [_P]),
+    "spice_batch_size": (ctypes.c_int, [_P, _PI32]),
+    "spice_batch_net": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_P)]),
     "spice_hip_create_from_ell": (ctypes.c_int, [_I32, _PI32, _U64, _U64, _F32, _I32, _U64, _U32,
                                                  ctypes.POINTER(_P)]),
     "spice_hip_create_from_csr": (ctypes.c_int, [_I32, _PI64, _PI32, _U64, _U64, _F32, _I32, _U64, _U32,

@@ -1,4 +1,5 @@
 // C ABI of the gfx950 simulator (declared in include/spice_hip.h).
+#include "batch.h"
 #include "graph.h"
 #include "net.h"
 
@@ -20,6 +21,12 @@ net_iface * make_net_brunel_with_plasticity( net_params const &, csr_dev &&, hip
 net_iface * make_net_synth( net_params const &, csr_dev &&, hipStream_t );
 net_iface * make_net_brunel_3pop( net_params const &, csr_dev &&, hipStream_t );
 net_iface * make_net_brunel_3pop_stdp( net_params const &, csr_dev &&, hipStream_t );
+batch_iface * make_batch_vogels_abbott( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
+batch_iface * make_batch_brunel( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
+batch_iface * make_batch_brunel_with_plasticity( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
+batch_iface * make_batch_synth( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
+batch_iface * make_batch_brunel_3pop( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
+batch_iface * make_batch_brunel_3pop_stdp( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
 } // namespace spice::hipx
 
 using namespace spice;
@@ -28,6 +35,13 @@ using namespace spice::hipx;
 struct spice_net
 {
 	net_iface * impl;
+	batch_iface * owner = nullptr; // set for nets borrowed from a batch
+};
+
+struct spice_batch
+{
+	batch_iface * impl;
+	std::vector<spice_net *> handles; // borrowed per-net handles, released with the batch
 };
 
 static thread_local std::string g_err;
@@ -126,10 +140,48 @@ static net_iface * make_net( net_params const & p, csr_dev && g, hipStream_t s )
 	}
 }
 
+// Queries and exports of a net borrowed from a batch first wait for the batch's stream.
 static net_iface * impl( spice_net * n )
 {
 	require( n && n->impl, "null net handle" );
+	if( n->owner ) n->owner->synchronize();
 	return n->impl;
+}
+
+// Stepping entry points: a net inside a batch is stepped only through the batch.
+static net_iface * stepping_impl( spice_net * n )
+{
+	require( n && n->impl, "null net handle" );
+	require( n->owner == nullptr, "a net that belongs to a batch is stepped through spice_batch_run" );
+	return n->impl;
+}
+
+static batch_iface * make_batch( int model, std::vector<net_params> const & ps, std::vector<csr_dev> && gs, hipStream_t s )
+{
+	try
+	{
+		switch( model )
+		{
+		case SPICE_MODEL_VOGELS_ABBOTT: return make_batch_vogels_abbott( ps, std::move( gs ), s );
+		case SPICE_MODEL_BRUNEL: return make_batch_brunel( ps, std::move( gs ), s );
+		case SPICE_MODEL_BRUNEL_WITH_PLASTICITY: return make_batch_brunel_with_plasticity( ps, std::move( gs ), s );
+		case SPICE_MODEL_SYNTH: return make_batch_synth( ps, std::move( gs ), s );
+		case SPICE_MODEL_BRUNEL_3POP: return make_batch_brunel_3pop( ps, std::move( gs ), s );
+		case SPICE_MODEL_BRUNEL_3POP_STDP: return make_batch_brunel_3pop_stdp( ps, std::move( gs ), s );
+		default: throw std::invalid_argument( "unknown model id" );
+		}
+	}
+	catch( ... )
+	{
+		(void)hipStreamDestroy( s );
+		throw;
+	}
+}
+
+static net_params params_of( int32_t model, spice_net_desc const & d, float dt, uint32_t flags )
+{
+	net_params p{ model, dt, d.delay, d.seed, flags, { d.params[0], d.params[1], d.params[2], d.params[3] } };
+	return p;
 }
 
 extern "C" {
@@ -203,7 +255,103 @@ int spice_hip_create(
 			(void)hipStreamDestroy( s );
 			throw;
 		}
-		*out = new spice_net{ make_net( { model, dt, delay, seed, flags }, std::move( g ), s ) };
+		*out = new spice_net{ make_net( { model, dt, delay, seed, flags, { 0, 0, 0, 0 } }, std::move( g ), s ) };
+	} );
+}
+
+int spice_hip_create_desc( int32_t model, spice_net_desc const * d, float dt, uint32_t flags, spice_net ** out )
+{
+	return guard( [&] {
+		require( out != nullptr && d != nullptr, "null argument" );
+		auto l = make_layout( d->pops, d->npops, d->conns, d->nconns );
+		require( dt > 0.0f, "dt must be positive" );
+		require( d->delay >= 1, "delay must be >= 1" );
+		hipStream_t s = new_stream();
+		csr_dev g;
+		try
+		{
+			g = generate_csr( gen_conns( l ), int64_t( l.size() ), int64_t( l.max_degree() ), d->seed, s );
+		}
+		catch( ... )
+		{
+			(void)hipStreamDestroy( s );
+			throw;
+		}
+		*out = new spice_net{ make_net( params_of( model, *d, dt, flags ), std::move( g ), s ) };
+	} );
+}
+
+int spice_batch_create( int32_t model, int32_t nnets, spice_net_desc const * descs, float dt, uint32_t flags, spice_batch ** out )
+{
+	return guard( [&] {
+		require( out != nullptr && descs != nullptr && nnets > 0, "invalid batch arguments" );
+		require( dt > 0.0f, "dt must be positive" );
+		std::vector<net_params> ps;
+		std::vector<util::layout> ls;
+		for( int32_t k = 0; k < nnets; k++ )
+		{
+			ls.push_back( make_layout( descs[k].pops, descs[k].npops, descs[k].conns, descs[k].nconns ) );
+			require( descs[k].delay >= 1, "delay must be >= 1" );
+			ps.push_back( params_of( model, descs[k], dt, flags ) );
+		}
+		hipStream_t s = new_stream();
+		std::vector<csr_dev> gs;
+		try
+		{
+			for( int32_t k = 0; k < nnets; k++ )
+				gs.push_back( generate_csr( gen_conns( ls[k] ), int64_t( ls[k].size() ), int64_t( ls[k].max_degree() ), descs[k].seed, s ) );
+		}
+		catch( ... )
+		{
+			(void)hipStreamDestroy( s );
+			throw;
+		}
+		auto * b = new spice_batch{ make_batch( model, ps, std::move( gs ), s ), {} };
+		for( int k = 0; k < b->impl->size(); k++ ) b->handles.push_back( new spice_net{ b->impl->net( k ), b->impl } );
+		*out = b;
+	} );
+}
+
+int spice_batch_destroy( spice_batch * b )
+{
+	return guard( [&] {
+		if( !b ) return;
+		for( auto * h : b->handles ) delete h;
+		delete b->impl;
+		delete b;
+	} );
+}
+
+int spice_batch_run( spice_batch * b, int64_t n )
+{
+	return guard( [&] {
+		require( b && b->impl, "null batch handle" );
+		b->impl->run( n );
+	} );
+}
+
+int spice_batch_synchronize( spice_batch * b )
+{
+	return guard( [&] {
+		require( b && b->impl, "null batch handle" );
+		b->impl->synchronize();
+	} );
+}
+
+int spice_batch_size( spice_batch * b, int32_t * n )
+{
+	return guard( [&] {
+		require( b && b->impl && n, "null argument" );
+		*n = b->impl->size();
+	} );
+}
+
+int spice_batch_net( spice_batch * b, int32_t i, spice_net ** out )
+{
+	return guard( [&] {
+		require( b && b->impl && out, "null argument" );
+		require( i >= 0 && i < int32_t( b->handles.size() ), "net index out of range" );
+		*out = b->handles[i];
 	} );
 }
 
@@ -242,7 +390,7 @@ int spice_hip_create_from_csr(
 			(void)hipStreamDestroy( s );
 			throw;
 		}
-		*out = new spice_net{ make_net( { model, dt, delay, seed, flags }, std::move( g ), s ) };
+		*out = new spice_net{ make_net( { model, dt, delay, seed, flags, { 0, 0, 0, 0 } }, std::move( g ), s ) };
 	} );
 }
 
@@ -270,6 +418,7 @@ int spice_hip_destroy( spice_net * net )
 {
 	return guard( [&] {
 		if( !net ) return;
+		require( net->owner == nullptr, "a net that belongs to a batch is released with the batch" );
 		delete net->impl;
 		delete net;
 	} );
@@ -277,7 +426,7 @@ int spice_hip_destroy( spice_net * net )
 
 int spice_hip_step( spice_net * net, int32_t ** dspk, uint32_t ** dcnt )
 {
-	return guard( [&] { impl( net )->step( dspk, dcnt ); } );
+	return guard( [&] { stepping_impl( net )->step( dspk, dcnt ); } );
 }
 int spice_hip_copy_spikes( spice_net * net, int32_t * out, uint64_t cap, uint64_t * count )
 {
@@ -288,13 +437,13 @@ int spice_hip_copy_spikes( spice_net * net, int32_t * out, uint64_t cap, uint64_
 }
 int spice_hip_run( spice_net * net, int64_t n )
 {
-	return guard( [&] { impl( net )->run( n ); } );
+	return guard( [&] { stepping_impl( net )->run( n ); } );
 }
 int spice_hip_run_profiled( spice_net * net, int64_t n, double * kernel_ms )
 {
 	return guard( [&] {
 		require( kernel_ms != nullptr, "null output" );
-		impl( net )->run_profiled( n, kernel_ms );
+		stepping_impl( net )->run_profiled( n, kernel_ms );
 	} );
 }
 int spice_hip_synchronize( spice_net * net )
@@ -336,7 +485,7 @@ int spice_hip_get_clock( spice_net * net, int64_t * step, float * sum, float * c
 }
 int spice_hip_set_clock( spice_net * net, int64_t step, float sum, float c )
 {
-	return guard( [&] { impl( net )->set_clock( step, sum, c ); } );
+	return guard( [&] { stepping_impl( net )->set_clock( step, sum, c ); } );
 }
 int spice_hip_get_stats( spice_net * net, spice_stats * out )
 {

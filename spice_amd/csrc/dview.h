@@ -61,6 +61,7 @@ struct dview
 	std::int32_t fparts; // work items per flushed row
 	std::uint32_t flags;
 	std::uint64_t seed;
+	float param[4]; // per-net model parameters (snn_info::param)
 
 	std::int64_t const * row_ptr;
 	std::int32_t const * col;

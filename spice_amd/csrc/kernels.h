@@ -155,14 +155,16 @@ __device__ inline snn_info info_of( dview const & v )
 {
 	snn_info i;
 	i.num_neurons = v.N;
+	for( int k = 0; k < 4; k++ ) i.param[k] = v.param[k];
 	return i;
 }
 __device__ inline float step_dt( dview const & v ) { return v.DT - v.ctl->c; }
 
-// Adds per-lane counts to this block's statistics slot: wave reduction, then the waves'
-// totals through LDS, then one plain read-add-write by thread 0. Every thread must call it.
+// Adds per-lane counts to statistics slot `slot` (a slot is written by one workgroup at a
+// time): wave reduction, then the waves' totals through LDS, then one plain read-add-write
+// by thread 0. Every thread of the block must call it.
 template <int K>
-__device__ inline void block_add_stats( dview const & v, unsigned long long const ( &x )[K], int const ( &idx )[K] )
+__device__ inline void block_add_stats( dview const & v, std::int64_t slot, unsigned long long const ( &x )[K], int const ( &idx )[K] )
 {
 	__shared__ unsigned long long part[16][K]; // blockDim.x <= 1024: at most 16 waves
 	int const w = threadIdx.x >> 6;
@@ -181,7 +183,7 @@ __device__ inline void block_add_stats( dview const & v, unsigned long long cons
 		{
 			unsigned long long t = 0;
 			for( int i = 0; i < nw; i++ ) t += part[i][k];
-			if( t ) v.bstats[std::int64_t( blockIdx.x ) * NSTAT + idx[k]] += t;
+			if( t ) v.bstats[slot * NSTAT + idx[k]] += t;
 		}
 	}
 }
@@ -467,7 +469,7 @@ __device__ inline std::int32_t segment_of( std::int32_t const * pre, std::int32_
 // atomics; at the end the tile's sums go to the global accumulators with plain stores:
 // one workgroup owns each tile, so no global atomics are needed.
 template <typename Model, bool FIXED>
-__global__ __launch_bounds__( TBS ) void k_receive_tiled( dview v )
+__device__ inline void receive_tile( dview const & v, std::int32_t tile )
 {
 	using N = typename Model::neuron;
 	using S = typename Model::synapse;
@@ -493,8 +495,6 @@ __global__ __launch_bounds__( TBS ) void k_receive_tiled( dview v )
 	snn_info const info = info_of( v );
 	int const tid = threadIdx.x;
 	unsigned long long ev = 0, pev = 0;
-
-	for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x )
 	{
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
@@ -597,8 +597,26 @@ __global__ __launch_bounds__( TBS ) void k_receive_tiled( dview v )
 		} );
 		__syncthreads();
 	}
-	block_add_stats<2>( v, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
+	block_add_stats<2>( v, tile, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
 }
+
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( TBS ) void k_receive_tiled( dview v )
+{
+	for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x ) receive_tile<Model, FIXED>( v, tile );
+}
+
+// Batched delivery: global tile gt belongs to net tile_net[gt] (local tile tile_local[gt]);
+// the net's view is read from device memory.
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( TBS ) void k_batch_receive(
+    dview const * views, std::int32_t const * tile_net, std::int32_t const * tile_local, std::int32_t ntiles )
+{
+	for( std::int32_t gt = blockIdx.x; gt < ntiles; gt += gridDim.x )
+		receive_tile<Model, FIXED>( views[tile_net[gt]], tile_local[gt] );
+}
+
+
 
 // Brings the rows queued by this step's neuron update up to date (lazy plasticity), one
 // workgroup per target tile: the queued rows' plastic segments inside the tile are walked as
@@ -690,11 +708,11 @@ __global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
 			}
 		}
 	block_add_stats<3>(
-	    v, { touched, slots, blockIdx.x == 0 && tid == 0 ? std::uint64_t( nrows ) : 0ull }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	    v, blockIdx.x, { touched, slots, blockIdx.x == 0 && tid == 0 ? std::uint64_t( nrows ) : 0ull }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
 template <typename Model, bool FIXED>
-__global__ __launch_bounds__( 256 ) void k_update( dview v )
+__device__ inline void update_block( dview const & v, std::int64_t blk )
 {
 	using N = typename Model::neuron;
 	constexpr bool plastic = is_plastic_v<Model>;
@@ -703,7 +721,7 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 	std::int64_t const T = v.ctl->step;
 	int const D1 = v.D + 1;
 	int const wslot = int( T % D1 );
-	if( blockIdx.x == 0 && threadIdx.x == 0 )
+	if( blk == 0 && threadIdx.x == 0 )
 	{
 		v.counts[( T + 1 ) % D1] = 0; // consumed by this step's receive; refilled at T + 1
 		if constexpr( plastic ) v.fl_count[( T + 1 ) & 1] = 0;
@@ -717,7 +735,7 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 	}
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
-	std::int64_t const i = std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+	std::int64_t const i = blk * 256 + threadIdx.x;
 	bool spiked = false, flush = false;
 	std::int32_t fl_synced = 0, fl_pend = 0;
 
@@ -826,7 +844,7 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 		}
 		bbase[0] = t0 ? atomicAdd( &v.counts[wslot], t0 ) : 0u;
 		if constexpr( plastic ) bbase[1] = t1 ? atomicAdd( &v.fl_count[T & 1], t1 ) : 0u;
-		if( t0 ) v.bstats[std::int64_t( blockIdx.x ) * NSTAT + ST_SPIKES] += t0;
+		if( t0 ) v.bstats[blk * NSTAT + ST_SPIKES] += t0;
 	}
 	__syncthreads();
 	if( spiked ) v.ring[std::int64_t( wslot ) * v.N + bbase[0] + wcount[w][0] + lanes_below( m )] = std::int32_t( i );
@@ -840,6 +858,19 @@ __global__ __launch_bounds__( 256 ) void k_update( dview v )
 			v.fl_pend[o] = fl_pend;
 		}
 	}
+}
+
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( 256 ) void k_update( dview v )
+{
+	update_block<Model, FIXED>( v, blockIdx.x );
+}
+
+// Batched neuron update: workgroup b updates block block_local[b] of net block_net[b].
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( 256 ) void k_batch_update( dview const * views, std::int32_t const * block_net, std::int32_t const * block_local )
+{
+	update_block<Model, FIXED>( views[block_net[blockIdx.x]], block_local[blockIdx.x] );
 }
 
 template <typename Model, bool FIXED>
@@ -909,7 +940,7 @@ __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 			if( p == 0 ) rows++;
 		}
 	}
-	block_add_stats<3>( v, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	block_add_stats<3>( v, blockIdx.x, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
 // Brings every row with plastic synapses up to date through step `now` (= last completed

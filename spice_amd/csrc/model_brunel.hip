@@ -1,4 +1,5 @@
-// Instantiates the gfx950 kernels and host driver for the model 'brunel'.
+// Instantiates the gfx950 kernels and host drivers for the model 'brunel'.
+#include "batch.h"
 #include "net.h"
 
 #include <spice/models/brunel.h>
@@ -8,5 +9,10 @@ namespace spice::hipx
 net_iface * make_net_brunel( net_params const & p, csr_dev && g, hipStream_t s )
 {
 	return new hip_net<::spice::brunel>( p, std::move( g ), s );
+}
+
+batch_iface * make_batch_brunel( std::vector<net_params> const & ps, std::vector<csr_dev> && gs, hipStream_t s )
+{
+	return make_batch<::spice::brunel>( ps, std::move( gs ), s );
 }
 } // namespace spice::hipx

@@ -31,6 +31,7 @@ struct net_params
 	int delay;
 	std::uint64_t seed;
 	std::uint32_t flags;
+	float param[4];
 };
 
 // Model-independent interface used by the C ABI.
@@ -342,6 +343,20 @@ public:
 		if( _g.nnz ) SPICE_HIP( hipMemcpy( _sa[a].data(), csr.data(), csr.size(), hipMemcpyHostToDevice ) );
 	}
 
+	// ---- used by hip_batch (batched stepping of many nets) ----
+	dview const & view( int parity ) const { return _vs[parity & 1]; }
+	int num_tiles() const { return _v.ntiles; }
+	int num_update_blocks() const { return _grid_update; }
+	std::int64_t host_step() const { return _step; }
+	void advance_host( std::int64_t n )
+	{
+		for( std::int64_t k = 0; k < n; k++ )
+		{
+			_step++;
+			_clock.add( _p.dt );
+		}
+	}
+
 private:
 	net_params _p;
 	csr_dev _g;
@@ -445,6 +460,7 @@ private:
 		v.DT = _p.dt;
 		v.flags = _p.flags;
 		v.seed = _p.seed;
+		for( int k = 0; k < 4; k++ ) v.param[k] = _p.param[k];
 		v.row_ptr = _g.row_ptr.data();
 		v.col = _g.col.data();
 		for( int a = 0; a < MAXA; a++ )

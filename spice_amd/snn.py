@@ -68,23 +68,38 @@ class Layout:
         return pops_a, conns_a
 
 
-class Snn:
-    """One network simulated on the GPU (cuda::snn<Model> counterpart)."""
+def _desc(layout, delay, seed, params, keep):
+    """NetDesc for a layout; `keep` collects the ctypes buffers the descriptor points to."""
+    if not isinstance(layout, Layout):
+        layout = Layout(*layout) if isinstance(layout, tuple) else Layout(layout)
+    pops_a, conns_a = layout._c_args()
+    keep.extend([pops_a, conns_a])
+    p = list(params or [])[:4]
+    p += [0.0] * (4 - len(p))
+    return _lib.NetDesc(_u64p(pops_a), len(layout.pops), conns_a, len(layout.conns), int(delay), int(seed),
+                        (_c.c_float * 4)(*p))
 
-    def __init__(self, model, layout, dt, delay=1, seed=1337, fixed_point=False, _handle=None, flags=0):
+
+class Snn:
+    """One network simulated on the GPU (cuda::snn<Model> counterpart).
+
+    params: up to 4 per-network model parameters (snn_info::param; e.g. brunel_3pop's Poisson
+    rate in Hz), 0 = the model's default."""
+
+    def __init__(self, model, layout, dt, delay=1, seed=1337, fixed_point=False, _handle=None, flags=0, params=None,
+                 _borrowed=False):
         self.model = model
         self._mid = model_id(model)
         self._h = _c.c_void_p()
+        self._borrowed = _borrowed
         L = _lib.lib()
         if _handle is not None:
             self._h = _handle
         else:
-            if not isinstance(layout, Layout):
-                layout = Layout(*layout) if isinstance(layout, tuple) else Layout(layout)
-            pops_a, conns_a = layout._c_args()
-            _lib.check(L.spice_hip_create(self._mid, _u64p(pops_a), len(layout.pops), conns_a, len(layout.conns),
-                                          float(dt), int(delay), int(seed), self._flags(fixed_point) | int(flags),
-                                          _c.byref(self._h)))
+            keep = []
+            d = _desc(layout, delay, seed, params, keep)
+            _lib.check(L.spice_hip_create_desc(self._mid, _c.byref(d), float(dt), self._flags(fixed_point) | int(flags),
+                                               _c.byref(self._h)))
         self._nattrs = neuron_attrs(model)
         self._sattrs = synapse_attrs(model)
 
@@ -130,9 +145,9 @@ class Snn:
 
     # ---- lifetime ----
     def close(self):
-        if self._h:
+        if self._h and not self._borrowed:
             _lib.check(_lib.lib().spice_hip_destroy(self._h))
-            self._h = _c.c_void_p()
+        self._h = _c.c_void_p()
 
     def __del__(self):
         try:
@@ -157,6 +172,15 @@ class Snn:
         buf = np.empty(n, dtype=np.int32)
         cnt = _c.c_uint64()
         _lib.check(L.spice_hip_copy_spikes(self._h, buf.ctypes.data_as(_c.POINTER(_c.c_int32)), n, _c.byref(cnt)))
+        return buf[: cnt.value].copy()
+
+    def step_spikes_of_last(self):
+        """Spike ids of the most recent step (without stepping)."""
+        n = self.num_neurons()
+        buf = np.empty(n, dtype=np.int32)
+        cnt = _c.c_uint64()
+        _lib.check(_lib.lib().spice_hip_copy_spikes(self._h, buf.ctypes.data_as(_c.POINTER(_c.c_int32)), n,
+                                                    _c.byref(cnt)))
         return buf[: cnt.value].copy()
 
     def step_device(self):
@@ -266,3 +290,48 @@ class Snn:
         for a, (name, dt) in enumerate(self._sattrs):
             arr = np.ascontiguousarray(state[name], dtype=dt)
             _lib.check(_lib.lib().spice_hip_import_synapse_attr(self._h, a, arr.ctypes.data, arr.nbytes, int(ell)))
+
+
+class Batch:
+    """Many independent networks of one static model stepped together (one launch per step
+    phase for all of them). nets: list of (layout, delay, seed[, params]) tuples. net(i) is a
+    borrowed Snn view of network i (queries, exports and spike readback; no stepping)."""
+
+    def __init__(self, model, nets, dt, fixed_point=False, flags=0):
+        self.model = model
+        keep = []
+        descs = (_lib.NetDesc * len(nets))(*[_desc(n[0], n[1], n[2], n[3] if len(n) > 3 else None, keep) for n in nets])
+        self._h = _c.c_void_p()
+        _lib.check(_lib.lib().spice_batch_create(model_id(model), len(nets), descs, float(dt),
+                                                 Snn._flags(fixed_point) | int(flags), _c.byref(self._h)))
+        self._dt = dt
+        self._nets = []
+        for i in range(len(nets)):
+            h = _c.c_void_p()
+            _lib.check(_lib.lib().spice_batch_net(self._h, i, _c.byref(h)))
+            self._nets.append(Snn(model, None, dt, _handle=h, _borrowed=True))
+
+    def __len__(self):
+        return len(self._nets)
+
+    def net(self, i):
+        return self._nets[i]
+
+    def run(self, nsteps):
+        _lib.check(_lib.lib().spice_batch_run(self._h, int(nsteps)))
+
+    def synchronize(self):
+        _lib.check(_lib.lib().spice_batch_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            for n in self._nets:
+                n._h = _c.c_void_p()
+            _lib.check(_lib.lib().spice_batch_destroy(self._h))
+            self._h = _c.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

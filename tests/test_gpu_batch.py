@@ -1,0 +1,76 @@
+"""Batched stepping (many independent networks, one launch per step phase): every network's
+result equals the same network stepped alone (bit for bit in fixed-point mode), for uneven
+sizes, delays, seeds and per-network model parameters; checked through the C ABI."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from spice_amd import Batch, Layout, Snn
+from tests.helpers import assert_state_equal, three_pop_layout
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_nets(model, count, seed0):
+    rng = np.random.default_rng(seed0)
+    nets = []
+    for k in range(count):
+        if model == "vogels_abbott":
+            n = int(rng.integers(300, 4000))
+            lay = ([n], [(0, 0, 0.05)])
+        else:
+            n = int(rng.integers(300, 5000))
+            lay = three_pop_layout(n)
+        delay = int(rng.choice([1, 2, 5, 15]))
+        params = [float(rng.uniform(5, 30))] if model == "brunel_3pop" else None
+        nets.append((Layout(*lay), delay, seed0 * 1000 + k, params))
+    return nets
+
+
+@pytest.mark.parametrize("model", ["brunel_3pop", "vogels_abbott"])
+def test_batch_equals_networks_alone(gpu, model):
+    nets = _mixed_nets(model, 9, 3)
+    steps = 300  # graph replays plus eager remainder steps
+    b = Batch(model, nets, 1e-4, fixed_point=True)
+    b.run(steps)
+    b.synchronize()
+    for k, (lay, delay, seed, params) in enumerate(nets):
+        alone = Snn(model, lay, 1e-4, delay, seed=seed, fixed_point=True, params=params)
+        alone.run(steps)
+        bn = b.net(k)
+        assert bn.clock() == alone.clock()
+        assert_state_equal(bn.neurons(), alone.neurons(), f"net {k}")
+        sa, sb = alone.stats(), bn.stats()
+        assert sa["spikes"] == sb["spikes"] and sa["events"] == sb["events"], (k, sa, sb)
+        np.testing.assert_array_equal(np.sort(bn.step_spikes_of_last()), np.sort(alone.step_spikes_of_last()))
+        alone.close()
+    b.close()
+
+
+def test_batch_net_parameters_match_checker(gpu):
+    """A per-network Poisson rate (snn_info::param[0]) reaches the model identically on the GPU
+    (inside a batch) and in the CPU checker."""
+    lay = three_pop_layout(1500)
+    b = Batch("brunel_3pop", [(Layout(*lay), 15, 77, [7.5]), (Layout(*lay), 15, 78, [28.0])], 1e-4, fixed_point=True)
+    for k, (seed, rate) in enumerate([(77, 7.5), (78, 28.0)]):
+        h = O.CpuSnn("brunel_3pop", lay, 1e-4, 15, seed=seed, fixed_point=True)
+        h.set_params([rate])
+        h.run(200)
+        if k == 0:
+            b.run(200)
+        assert_state_equal(b.net(k).neurons(), h.neurons(), f"net {k}")
+    b.close()
+
+
+def test_borrowed_nets_cannot_be_stepped_or_destroyed(gpu):
+    b = Batch("brunel", [(Layout([200, 200], [(0, 1, 0.1), (1, 1, 0.1)]), 1, 1)], 1e-4)
+    n = b.net(0)
+    with pytest.raises(ValueError):
+        n.step()
+    with pytest.raises(ValueError):
+        n.run(5)
+    with pytest.raises(ValueError):
+        Batch("brunel_3pop_stdp", [(Layout(*three_pop_layout(500)), 1, 1)], 1e-4)
+    b.run(10)
+    assert n.clock()[0] == 10
+    b.close()
