@@ -18,6 +18,7 @@ the reference's cpu::snn, timed on this host on a bounded number of steps of the
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -76,6 +77,86 @@ def algorithmic_bytes(model, n, st, steps):
     return {"receive": recv, "update": upd, "flush": fl}
 
 
+def sweep_nets(count, seed):
+    """Parameter sweep of the batched configuration: `count` independent nets, alternately
+    Brunel (brunel_3pop, p = 0.1, delay 15, Poisson rate ~ U[5, 30] Hz) and Vogels-Abbott
+    (p = 0.02, delay 1), synapse counts log-uniform in [1e4, 1e8], seed = base + index."""
+    rng = np.random.default_rng(seed)
+    nets = []
+    for k in range(count):
+        syn = 10 ** rng.uniform(4, 8)
+        if k % 2 == 0:
+            n = max(20, int(round(np.sqrt(syn / 0.1))))
+            pops, conns = three_pop(n, 0.1)
+            nets.append(("brunel_3pop", pops, conns, 15, 1000 + k, [float(rng.uniform(5, 30))]))
+        else:
+            n = max(20, int(round(np.sqrt(syn / 0.02))))
+            nets.append(("vogels_abbott", [n], [(0, 0, 0.02)], 1, 1000 + k, None))
+    return nets
+
+
+def run_batched(count, steps, seq_stride):
+    """Aggregate synapse events/s of the sweep stepped as batches (one batch per model, the
+    two batches on their own streams) vs. the same nets stepped one at a time on the same GPU
+    (every seq_stride-th net, rate = their events / their summed wall time); per-net spike
+    counts of the two runs are compared: fp32 accumulation order differs between launch shapes,
+    so trajectories diverge and only statistics agree -- summed over the sample (2 % tolerance),
+    per net for nets with >= 1e4 spikes, and as a z-score |a-b|/sqrt(a+b) per net."""
+    from spice_amd import Batch, Layout, Snn
+    nets = sweep_nets(count, 7)
+    dt = 1e-4
+    groups = {}
+    for k, (m, pops, conns, d, seed, par) in enumerate(nets):
+        groups.setdefault(m, []).append(k)
+    t0 = time.perf_counter()
+    batches = {m: Batch(m, [(Layout(nets[k][1], nets[k][2]), nets[k][3], nets[k][4], nets[k][5]) for k in ks], dt)
+               for m, ks in groups.items()}
+    for b in batches.values():
+        b.synchronize()
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for b in batches.values():
+        b.run(steps)
+    for b in batches.values():
+        b.synchronize()
+    wall = time.perf_counter() - t0
+    events, spikes, syn = 0, {}, 0
+    for m, ks in groups.items():
+        for i, k in enumerate(ks):
+            st = batches[m].net(i).stats()
+            events += st["events"]
+            spikes[k] = st["spikes"]
+            syn += batches[m].net(i).num_edges()
+    for b in batches.values():
+        b.close()
+    # sequential plain runs of a sample
+    seq_events, seq_wall, rel = 0, 0.0, []
+    for k in range(0, count, seq_stride):
+        m, pops, conns, d, seed, par = nets[k]
+        net = Snn(m, Layout(pops, conns), dt, d, seed=seed, params=par)
+        net.synchronize()
+        t1 = time.perf_counter()
+        net.run(steps)
+        net.synchronize()
+        seq_wall += time.perf_counter() - t1
+        st = net.stats()
+        seq_events += st["events"]
+        rel.append((st["spikes"], spikes[k]))
+        net.close()
+    agg = events / wall
+    seq = seq_events / seq_wall
+    print(f"[bench] batched {count} nets ({syn:.3g} synapses) x {steps} steps: {wall:.3f}s, "
+          f"{agg:.3g} ev/s; sequential sample {seq:.3g} ev/s", file=sys.stderr, flush=True)
+    return {"nets": count, "synapses": syn, "steps": steps, "bio_s": steps * dt, "setup_s": round(setup, 3),
+            "wall_s": round(wall, 4), "syn_events_per_s": round(agg, 1),
+            "sequential_syn_events_per_s": round(seq, 1), "speedup_vs_sequential": round(agg / seq, 3),
+            "sequential_sample": f"every {seq_stride}th net ({len(range(0, count, seq_stride))} nets), same steps",
+            "spike_count_rel_diff_total": round(abs(sum(a for a, _ in rel) - sum(b for _, b in rel))
+                                                / max(1, sum(a for a, _ in rel)), 5),
+            "max_spike_count_rel_diff_1e4": round(max([abs(a - b) / a for a, b in rel if a >= 10_000] or [0]), 5),
+            "max_spike_count_z": round(max([abs(a - b) / math.sqrt(a + b) for a, b in rel if a + b > 0] or [0]), 3)}
+
+
 def cpu_baseline(model, lay, delay, seed, net, steps):
     """Times the CPU checker (single thread) on `steps` steps of the same network: the graph
     is copied from the GPU, state starts from the model's init (as the GPU run did)."""
@@ -103,6 +184,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--flags", type=int, default=0, help="spice_flag launch-shape bits (see include/spice_hip.h)")
+    ap.add_argument("--batch-nets", type=int, default=512, help="nets of the batched sweep (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--batch-seq-stride", type=int, default=8)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -159,6 +243,9 @@ def main():
                          f"state from init; dense per-step plasticity as in cpu::snn), "
                          f"{s_per_step:.2f} s/step, wall/bio extrapolated linearly; init {t_init:.1f}s excluded"}
 
+    net.close()
+    batched = run_batched(args.batch_nets, args.batch_steps, args.batch_seq_stride) if args.batch_nets > 0 else None
+
     out = {
         "metric": METRIC,
         "value": round(wall / bio, 5),
@@ -180,9 +267,9 @@ def main():
         "setup_s": round(setup_s, 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
-    net.close()
 
 
 if __name__ == "__main__":
