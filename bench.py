@@ -157,6 +157,101 @@ def run_batched(count, steps, seq_stride):
             "max_spike_count_z": round(max([abs(a - b) / math.sqrt(a + b) for a, b in rel if a + b > 0] or [0]), 3)}
 
 
+def serve_stream(count, rate, seed):
+    """Served job stream: Poisson arrivals at `rate` jobs/s; 40 % Brunel (brunel_3pop, p = 0.1,
+    delay 15), 40 % Vogels-Abbott (p = 0.02, delay 1), 20 % Brunel+STDP (brunel_3pop_stdp,
+    p = 0.1, delay 15); synapse counts log-uniform in [1e5, 2e9], biological duration uniform
+    in [0.1, 10] s at dt = 0.1 ms, random seeds."""
+    rng = np.random.default_rng(seed)
+    jobs, t = [], 0.0
+    for _ in range(count):
+        t += rng.exponential(1.0 / rate)
+        u = rng.random()
+        syn = float(np.exp(rng.uniform(np.log(1e5), np.log(2e9))))
+        steps = int(round(rng.uniform(0.1, 10.0) / 1e-4))
+        job_seed = int(rng.integers(1, 2**31))
+        if u < 0.8 and u >= 0.4:
+            n = int(round(np.sqrt(syn / 0.02)))
+            jobs.append((t, "vogels_abbott", [n], [(0, 0, 0.02)], 1, steps, job_seed))
+        else:
+            n = int(round(np.sqrt(syn / 0.1)))
+            pops, conns = three_pop(n, 0.1)
+            jobs.append((t, "brunel_3pop" if u < 0.4 else "brunel_3pop_stdp", pops, conns, 15, steps, job_seed))
+    return jobs
+
+
+def run_serving(count, rate, budget, chunk, seq_stride, seed):
+    """Continuous batching of a Poisson job stream through the library's server (admission
+    against the HBM budget, device-resident jobs folded into batched chunks, spikes streamed
+    back per chunk) vs the same jobs run one at a time as plain networks (a sample: every
+    seq_stride-th job). Latency = submit -> last spikes delivered."""
+    from spice_amd import Layout, Server, Snn
+    jobs = serve_stream(count, rate, seed)
+    streamed = [0]
+
+    def sink(job, first, off, ids):
+        streamed[0] += len(ids)
+
+    sv = Server(budget_bytes=budget, chunk_steps=chunk, on_spikes=sink)
+    t0 = time.perf_counter()
+    ids = []
+    for (t, m, pops, conns, d, steps, js) in jobs:
+        lag = t - (time.perf_counter() - t0)
+        if lag > 0:
+            time.sleep(lag)
+        ids.append(sv.submit(m, Layout(pops, conns), 1e-4, steps, delay=d, seed=js))
+    recs = [sv.wait(i) for i in ids]
+    makespan = time.perf_counter() - t0
+    info = sv.info()
+    sv.close()
+    failed = [r for r in recs if r["status"] != "done"]
+    lat = np.array([r["t_done"] - r["t_submit"] for r in recs])
+    events = sum(r["events"] for r in recs)
+    syn = sum(r["synapses"] for r in recs)
+    print(f"[bench] served {count} jobs ({syn:.3g} synapses, {events:.3g} events) in {makespan:.1f}s: "
+          f"p50 {np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s, {len(failed)} not done",
+          file=sys.stderr, flush=True)
+    # sequential plain runs of a sample of the same jobs
+    seq_events, seq_run, seq_setup, rel = 0, 0.0, 0.0, []
+    for k in range(0, count, seq_stride):
+        _, m, pops, conns, d, steps, js = jobs[k]
+        t1 = time.perf_counter()
+        net = Snn(m, Layout(pops, conns), 1e-4, d, seed=js)
+        net.synchronize()
+        t2 = time.perf_counter()
+        net.run(steps)
+        net.synchronize()
+        t3 = time.perf_counter()
+        st = net.stats()
+        net.close()
+        seq_setup += t2 - t1
+        seq_run += t3 - t2
+        seq_events += st["events"]
+        rel.append((st["spikes"], recs[k]["spikes"]))
+    seq = seq_events / seq_run
+    seq_incl = seq_events / (seq_run + seq_setup)
+    served = events / makespan
+    return {
+        "jobs": count, "arrival_rate_per_s": rate, "budget_bytes": budget, "chunk_steps": chunk,
+        "mix": "40% brunel_3pop / 40% vogels_abbott / 20% brunel_3pop_stdp, synapses log-uniform 1e5-2e9, "
+               "bio U[0.1, 10] s",
+        "synapses_total": syn, "events_total": events, "spikes_streamed": streamed[0],
+        "makespan_s": round(makespan, 3), "busy_s": round(info["busy_s"], 3),
+        "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+        "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+        "latency_max_s": round(float(lat.max()), 4),
+        "syn_events_per_s": round(served, 1),
+        "sequential_syn_events_per_s": round(seq, 1),
+        "sequential_syn_events_per_s_incl_setup": round(seq_incl, 1),
+        "speedup_vs_sequential": round(served / seq, 3),
+        "sequential_sample": f"every {seq_stride}th job ({len(range(0, count, seq_stride))} jobs), plain Snn, same seeds",
+        "peak_reserved_bytes": info["peak_reserved"], "chunks": info["chunks"],
+        "not_done": len(failed),
+        "spike_count_rel_diff_total": round(abs(sum(a for a, _ in rel) - sum(b for _, b in rel))
+                                            / max(1, sum(a for a, _ in rel)), 5),
+    }
+
+
 def cpu_baseline(model, lay, delay, seed, net, steps):
     """Times the CPU checker (single thread) on `steps` steps of the same network: the graph
     is copied from the GPU, state starts from the model's init (as the GPU run did)."""
@@ -187,6 +282,11 @@ def main():
     ap.add_argument("--batch-nets", type=int, default=512, help="nets of the batched sweep (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-seq-stride", type=int, default=8)
+    ap.add_argument("--serve-jobs", type=int, default=64, help="jobs of the served stream (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, jobs per second")
+    ap.add_argument("--serve-budget", type=float, default=250e9, help="HBM admission budget, bytes")
+    ap.add_argument("--serve-chunk", type=int, default=1000, help="steps per serving chunk")
+    ap.add_argument("--serve-seq-stride", type=int, default=8)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -245,6 +345,8 @@ def main():
 
     net.close()
     batched = run_batched(args.batch_nets, args.batch_steps, args.batch_seq_stride) if args.batch_nets > 0 else None
+    served = (run_serving(args.serve_jobs, args.serve_rate, args.serve_budget, args.serve_chunk,
+                          args.serve_seq_stride, 11) if args.serve_jobs > 0 else None)
 
     out = {
         "metric": METRIC,
@@ -268,6 +370,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

@@ -226,8 +226,8 @@ int spice_hip_synchronize( spice_net * net );
 
 /* ---- batches: many independent networks stepped together -------------------------------- */
 /* No counterpart in the reference (it steps one network per cuda::snn). A batch holds nnets
- * networks of one static model (sizes, delays, seeds and parameters may differ; dt and flags
- * are shared) and steps all of them with one launch per step phase (a captured HIP graph).
+ * networks of one model (sizes, delays, seeds and parameters may differ; dt and flags are
+ * shared) and steps all of them with one launch per step phase (a captured HIP graph).
  * Each network's results equal those of the same network stepped alone. */
 int spice_batch_create( int32_t model, int32_t nnets, const spice_net_desc * descs, float dt, uint32_t flags, spice_batch ** out_batch );
 int spice_batch_destroy( spice_batch * batch );
@@ -239,6 +239,78 @@ int spice_batch_size( spice_batch * batch, int32_t * out_nnets );
  * readback of this header works on it (they wait for the batch first); stepping it alone,
  * setting its clock or destroying it is rejected. */
 int spice_batch_net( spice_batch * batch, int32_t i, spice_net ** out_net );
+
+/* ---- serving: a request loop over device-resident simulation jobs ------------------------ */
+/* No counterpart in the reference. A server admits submitted jobs in arrival order while the
+ * sum of their device-memory reservations fits its budget, keeps each admitted job's graph
+ * and state on the device until it has run its steps, and steps all resident jobs together
+ * in chunks (one batched launch sequence per model and accumulation mode, the models on
+ * their own streams); jobs arriving while others run join at the next chunk. After every
+ * chunk the spikes of each job are handed to the sink, on the server's worker thread:
+ * offsets[k] .. offsets[k+1] index the ids of step first_step + k (nsteps + 1 offsets);
+ * the arrays are valid during the call only. A job's spikes and statistics equal those of
+ * the same network run alone (bit-identical with SPICE_FLAG_FIXED_POINT). */
+typedef struct spice_server spice_server;
+typedef void ( *spice_spike_sink )(
+    void * user, uint64_t job, int64_t first_step, int64_t nsteps, const uint64_t * offsets, const int32_t * ids );
+
+typedef struct spice_job_desc
+{
+	int32_t model;      /* spice_model_id */
+	spice_net_desc net; /* populations, connections, delay, seed, params (read at submit) */
+	float dt;
+	uint32_t flags;
+	int64_t steps; /* duration in steps (>= 1) */
+} spice_job_desc;
+
+typedef enum spice_job_status
+{
+	SPICE_JOB_QUEUED = 0,
+	SPICE_JOB_RUNNING = 1,
+	SPICE_JOB_DONE = 2,
+	SPICE_JOB_FAILED = 3,
+	SPICE_JOB_CANCELLED = 4
+} spice_job_status;
+
+typedef struct spice_job_result
+{
+	uint64_t id;
+	int32_t status; /* spice_job_status */
+	int32_t reserved0;
+	int64_t steps, done; /* requested / simulated */
+	uint64_t synapses, device_bytes, reserved_bytes;
+	uint64_t spikes, events, plastic_events;
+	/* wrapping sum over the streamed spikes of mix64(step << 32 | id), mix64 = the splitmix64
+	 * finalizer: equal digests <=> (with overwhelming probability) equal spike streams */
+	uint64_t digest;
+	double t_submit, t_admit, t_done; /* seconds since the server started */
+} spice_job_result;
+
+typedef struct spice_server_info
+{
+	uint64_t budget, reserved, peak_reserved;
+	int64_t queued, resident, finished, chunks, rebuilds, steps_launched;
+	double busy_s, now;
+} spice_server_info;
+
+/* budget_bytes: device-memory admission budget; chunk_steps: longest chunk (also the spike
+ * log window). sink may be NULL. */
+int spice_server_create( uint64_t budget_bytes, int64_t chunk_steps, spice_spike_sink sink, void * user, spice_server ** out );
+/* Cancels queued and running jobs, waits for the worker and frees everything. */
+int spice_server_destroy( spice_server * server );
+/* Device-memory reservation of a job for a server with chunks of chunk_steps steps. */
+int spice_server_estimate( const spice_job_desc * job, int64_t chunk_steps, uint64_t * out_bytes );
+/* Rejects (SPICE_ERR_INVALID_ARGUMENT) a job whose reservation exceeds the budget. */
+int spice_server_submit( spice_server * server, const spice_job_desc * job, uint64_t * out_id );
+/* Waits up to timeout_s seconds (negative: no limit) for the job to finish; *out_finished
+ * tells whether it did, *out gets its record either way. */
+int spice_server_wait( spice_server * server, uint64_t id, double timeout_s, spice_job_result * out, int32_t * out_finished );
+int spice_server_cancel( spice_server * server, uint64_t id );
+/* Waits until no job is queued or resident. */
+int spice_server_drain( spice_server * server );
+int spice_server_get_info( spice_server * server, spice_server_info * out );
+/* Error message of a failed job (empty otherwise), NUL-terminated, truncated to cap. */
+int spice_server_job_error( spice_server * server, uint64_t id, char * buf, uint64_t cap );
 
 /* ---- queries ---------------------------------------------------------------------------- */
 

@@ -278,6 +278,17 @@ class CpuSnn:
     def run(self, nsteps):
         _check(lib().oracle_run(self._h, _c.c_int64(nsteps)))
 
+    def run_record(self, nsteps, cap=None):
+        """Runs nsteps steps in one call (the GIL is released meanwhile); returns (offsets,
+        ids): the spikes of step k (counted from the current step) are ids[offsets[k]:offsets[k+1]]."""
+        if cap is None:
+            cap = self.num_neurons() * nsteps // 16 + 4096
+        ids = np.empty(max(cap, 1), np.int32)
+        off = np.empty(nsteps + 1, np.uint64)
+        _check(lib().oracle_run_record(self._h, _c.c_int64(nsteps), _c.c_void_p(ids.ctypes.data), _c.c_uint64(cap),
+                                       _c.c_void_p(off.ctypes.data)))
+        return off, ids[: int(off[-1])].copy()
+
     def csr(self):
         n, _, nnz = self._info()
         rp = np.empty(n + 1, np.int64)

@@ -198,6 +198,25 @@ int oracle_step( oracle::net_base * net, int32_t * out, uint64_t cap, uint64_t *
 	} );
 }
 
+// Runs nsteps steps and records every step's spikes: ids of step k at
+// ids[offsets[k] .. offsets[k+1]) (nsteps + 1 offsets).
+int oracle_run_record( oracle::net_base * net, int64_t nsteps, int32_t * ids, uint64_t cap, uint64_t * offsets )
+{
+	return guard( [&] {
+		std::vector<int> s;
+		uint64_t o = 0;
+		for( int64_t i = 0; i < nsteps; i++ )
+		{
+			offsets[i] = o;
+			net->step( &s );
+			if( o + s.size() > cap ) throw std::invalid_argument( "spike buffer too small" );
+			std::memcpy( ids + o, s.data(), s.size() * 4 );
+			o += s.size();
+		}
+		offsets[nsteps] = o;
+	} );
+}
+
 int oracle_run( oracle::net_base * net, int64_t nsteps )
 {
 	return guard( [&] {

@@ -47,6 +47,31 @@ class Stats(ctypes.Structure):
                 ("steps", "spikes", "events", "plastic_events", "flush_rows", "flush_synapses", "flush_slots")]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int32), ("net", NetDesc), ("dt", ctypes.c_float), ("flags", ctypes.c_uint32),
+                ("steps", ctypes.c_int64)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("steps", ctypes.c_int64), ("done", ctypes.c_int64),
+                ("synapses", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64), ("reserved_bytes", ctypes.c_uint64),
+                ("spikes", ctypes.c_uint64), ("events", ctypes.c_uint64), ("plastic_events", ctypes.c_uint64),
+                ("digest", ctypes.c_uint64),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [("budget", ctypes.c_uint64), ("reserved", ctypes.c_uint64), ("peak_reserved", ctypes.c_uint64),
+                ("queued", ctypes.c_int64), ("resident", ctypes.c_int64), ("finished", ctypes.c_int64),
+                ("chunks", ctypes.c_int64), ("rebuilds", ctypes.c_int64), ("steps_launched", ctypes.c_int64),
+                ("busy_s", ctypes.c_double), ("now", ctypes.c_double)]
+
+
+SPIKE_SINK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
+                              ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32))
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
+
 _P = ctypes.c_void_p
 _I32, _U32, _I64, _U64, _F32 = ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
 _PI32, _PU32, _PI64, _PU64, _PF32 = (ctypes.POINTER(t) for t in (_I32, _U32, _I64, _U64, _F32))
@@ -67,6 +92,15 @@ SIGNATURES = {
     "spice_batch_synchronize": (ctypes.c_int, [_P]),
     "spice_batch_size": (ctypes.c_int, [_P, _PI32]),
     "spice_batch_net": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_P)]),
+    "spice_server_create": (ctypes.c_int, [_U64, _I64, SPIKE_SINK, _P, ctypes.POINTER(_P)]),
+    "spice_server_destroy": (ctypes.c_int, [_P]),
+    "spice_server_estimate": (ctypes.c_int, [ctypes.POINTER(JobDesc), _I64, _PU64]),
+    "spice_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(JobDesc), _PU64]),
+    "spice_server_wait": (ctypes.c_int, [_P, _U64, ctypes.c_double, ctypes.POINTER(JobResult), _PI32]),
+    "spice_server_cancel": (ctypes.c_int, [_P, _U64]),
+    "spice_server_drain": (ctypes.c_int, [_P]),
+    "spice_server_get_info": (ctypes.c_int, [_P, ctypes.POINTER(ServerInfo)]),
+    "spice_server_job_error": (ctypes.c_int, [_P, _U64, ctypes.c_char_p, _U64]),
     "spice_hip_create_from_ell": (ctypes.c_int, [_I32, _PI32, _U64, _U64, _F32, _I32, _U64, _U32,
                                                  ctypes.POINTER(_P)]),
     "spice_hip_create_from_csr": (ctypes.c_int, [_I32, _PI64, _PI32, _U64, _U64, _F32, _I32, _U64, _U32,

@@ -2,9 +2,11 @@
 // Batched stepping of many independent networks of one model (parameter sweeps, served
 // jobs). Every net keeps its own buffers, clock and statistics (it is an ordinary hip_net);
 // the batch adds device tables that map the workgroups of ONE launch per step phase to
-// (net, local tile) and (net, local neuron block), and replays a captured HIP graph of those
-// launches. Per-net results are identical to stepping the nets one by one: the kernels run the
-// same per-net bodies and all randomness is keyed on (net seed, neuron, step).
+// (net, local tile), (net, local neuron block) and, for plastic models, (net, local flush
+// block), and replays a captured HIP graph of those launches. Per-net results are identical
+// to stepping the nets one by one: the kernels run the same per-net bodies and all randomness
+// is keyed on (net seed, neuron, step). Nets may sit at different steps (served jobs join a
+// running mix): each net's clock slot is chosen from its own step parity.
 #include "net.h"
 
 #include <memory>
@@ -19,54 +21,72 @@ struct batch_iface
 	virtual void synchronize() = 0;
 	virtual int size() const = 0;
 	virtual net_iface * net( int i ) = 0;
+	// opens the spike-log window of every net (with a log) over the next nsteps steps
+	virtual void begin_spike_log( std::int64_t nsteps ) = 0;
 };
 
 template <typename Model>
 class hip_batch final : public batch_iface
 {
 	static constexpr int graph_steps = 32;
+	static constexpr bool plastic = is_plastic_v<Model>;
 
 public:
-	hip_batch( std::vector<std::unique_ptr<hip_net<Model>>> && nets, hipStream_t stream )
+	// `nets` must outlive the batch unless they are also handed over in `owned`. The batch
+	// destroys `stream` at the end if `own_stream`.
+	hip_batch( std::vector<hip_net<Model> *> nets, std::vector<std::unique_ptr<hip_net<Model>>> && owned, hipStream_t stream, bool own_stream )
 	    : _nets( std::move( nets ) )
+	    , _owned( std::move( owned ) )
 	    , _s( stream )
+	    , _own_stream( own_stream )
 	{
-		static_assert( !is_plastic_v<Model>, "batches hold static (non-plastic) models" );
 		if( _nets.empty() ) throw std::invalid_argument( "empty batch" );
-		_fixed = ( _nets[0]->params().flags & SPICE_FLAG_FIXED_POINT ) != 0;
-		for( auto & n : _nets )
-		{
-			if( n->host_step() != 0 ) throw std::invalid_argument( "batch nets must start at step 0" );
-			if( ( ( n->params().flags & SPICE_FLAG_FIXED_POINT ) != 0 ) != _fixed )
-				throw std::invalid_argument( "all nets of a batch use the same accumulation mode" );
-		}
-		std::vector<std::int32_t> tn, tl, bn, bl;
+		_fixed = _nets[0]->fixed();
+		for( auto * n : _nets )
+			if( n->fixed() != _fixed ) throw std::invalid_argument( "all nets of a batch use the same accumulation mode" );
+		std::vector<std::int32_t> tn, tl, bn, bl, fn, fl, fc;
 		std::vector<dview> v0, v1;
 		for( std::size_t k = 0; k < _nets.size(); k++ )
 		{
-			for( int t = 0; t < _nets[k]->num_tiles(); t++ )
+			hip_net<Model> & n = *_nets[k];
+			for( int t = 0; t < n.num_tiles(); t++ )
 			{
 				tn.push_back( std::int32_t( k ) );
 				tl.push_back( t );
 			}
-			for( int b = 0; b < _nets[k]->num_update_blocks(); b++ )
+			for( int b = 0; b < n.num_update_blocks(); b++ )
 			{
 				bn.push_back( std::int32_t( k ) );
 				bl.push_back( b );
 			}
-			v0.push_back( _nets[k]->view( 0 ) );
-			v1.push_back( _nets[k]->view( 1 ) );
+			if constexpr( plastic )
+			{
+				int const nf = n.num_batch_flush_blocks();
+				for( int b = 0; b < nf; b++ )
+				{
+					fn.push_back( std::int32_t( k ) );
+					fl.push_back( b );
+					fc.push_back( nf );
+				}
+			}
+			// batch step t runs net step host_step + t: clock slot (host_step + t) & 1
+			v0.push_back( n.view( int( n.host_step() & 1 ) ) );
+			v1.push_back( n.view( int( ( n.host_step() + 1 ) & 1 ) ) );
 		}
 		_ntiles = std::int32_t( tn.size() );
 		_nblocks = std::int32_t( bn.size() );
+		_nflush = std::int32_t( fn.size() );
 		upload( _tile_net, tn );
 		upload( _tile_local, tl );
 		upload( _block_net, bn );
 		upload( _block_local, bl );
+		upload( _flush_net, fn );
+		upload( _flush_local, fl );
+		upload( _flush_count, fc );
 		upload( _views[0], v0 );
 		upload( _views[1], v1 );
 		_grid_tiles = std::min( _ntiles, 16 * device_cu_count() );
-		for( auto & n : _nets ) n->synchronize();
+		for( auto * n : _nets ) n->synchronize();
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
@@ -74,7 +94,7 @@ public:
 	{
 		if( _exec ) (void)hipGraphExecDestroy( _exec );
 		if( _s ) (void)hipStreamSynchronize( _s );
-		if( _s ) (void)hipStreamDestroy( _s );
+		if( _s && _own_stream ) (void)hipStreamDestroy( _s );
 	}
 
 	void run( std::int64_t n ) override
@@ -96,7 +116,14 @@ public:
 			}
 		}
 		while( done < n ) eager();
-		for( auto & net : _nets ) net->advance_host( n );
+		for( auto * net : _nets ) net->advance_host( n );
+	}
+
+	void begin_spike_log( std::int64_t nsteps ) override
+	{
+		for( auto * net : _nets ) net->note_spike_log( nsteps ); // validates nsteps first
+		std::int32_t const n = std::int32_t( _nets.size() );
+		SPICE_LAUNCH( k_batch_log_window<<<unsigned( ( n + 255 ) / 256 ), 256, 0, _s>>>( _views[_step & 1].data(), n, nsteps ) );
 	}
 
 	void synchronize() override { SPICE_HIP( hipStreamSynchronize( _s ) ); }
@@ -104,18 +131,20 @@ public:
 	net_iface * net( int i ) override
 	{
 		if( i < 0 || i >= size() ) throw std::invalid_argument( "net index out of range" );
-		return _nets[i].get();
+		return _nets[i];
 	}
 
 private:
-	std::vector<std::unique_ptr<hip_net<Model>>> _nets;
+	std::vector<hip_net<Model> *> _nets;
+	std::vector<std::unique_ptr<hip_net<Model>>> _owned;
 	hipStream_t _s = nullptr;
+	bool _own_stream = false;
 	hipGraphExec_t _exec = nullptr;
 	bool _fixed = false;
 	std::int64_t _step = 0;
-	std::int32_t _ntiles = 0, _nblocks = 0;
+	std::int32_t _ntiles = 0, _nblocks = 0, _nflush = 0;
 	int _grid_tiles = 0;
-	dbuf<std::int32_t> _tile_net, _tile_local, _block_net, _block_local;
+	dbuf<std::int32_t> _tile_net, _tile_local, _block_net, _block_local, _flush_net, _flush_local, _flush_count;
 	dbuf<dview> _views[2];
 
 	template <typename T>
@@ -140,6 +169,8 @@ private:
 		dview const * v = _views[t & 1].data();
 		SPICE_LAUNCH( k_batch_receive<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v, _tile_net.data(), _tile_local.data(), _ntiles ) );
 		SPICE_LAUNCH( k_batch_update<Model, F><<<_nblocks, 256, 0, _s>>>( v, _block_net.data(), _block_local.data() ) );
+		if constexpr( plastic )
+			SPICE_LAUNCH( k_batch_flush<Model, F><<<_nflush, 256, 0, _s>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
 	}
 
 	void capture()
@@ -163,11 +194,13 @@ private:
 	}
 };
 
-// Builds the nets of a batch (each gets its own stream for setup) and the batch itself.
+// Builds the nets of a batch (each gets its own stream for setup) and the batch, which owns
+// them and `s`.
 template <typename Model>
 batch_iface * make_batch( std::vector<net_params> const & ps, std::vector<csr_dev> && gs, hipStream_t s )
 {
 	std::vector<std::unique_ptr<hip_net<Model>>> nets;
+	std::vector<hip_net<Model> *> raw;
 	for( std::size_t k = 0; k < ps.size(); k++ )
 	{
 		hipStream_t ns = nullptr;
@@ -181,7 +214,22 @@ batch_iface * make_batch( std::vector<net_params> const & ps, std::vector<csr_de
 			(void)hipStreamDestroy( ns );
 			throw;
 		}
+		raw.push_back( nets.back().get() );
 	}
-	return new hip_batch<Model>( std::move( nets ), s );
+	return new hip_batch<Model>( std::move( raw ), std::move( nets ), s, true );
+}
+
+// A batch over nets owned elsewhere (all of model `Model`); `s` stays the caller's.
+template <typename Model>
+batch_iface * make_group( std::vector<net_iface *> const & nets, hipStream_t s )
+{
+	std::vector<hip_net<Model> *> raw;
+	for( auto * n : nets )
+	{
+		auto * h = dynamic_cast<hip_net<Model> *>( n );
+		if( !h ) throw std::invalid_argument( "batch nets must all be of the batch's model" );
+		raw.push_back( h );
+	}
+	return new hip_batch<Model>( std::move( raw ), {}, s, false );
 }
 } // namespace spice::hipx

@@ -1,7 +1,6 @@
 // C ABI of the gfx950 simulator (declared in include/spice_hip.h).
-#include "batch.h"
-#include "graph.h"
-#include "net.h"
+#include "factory.h"
+#include "serve.h"
 
 #include <spice/models/all.h>
 #include <spice/util/layout.h>
@@ -13,21 +12,6 @@
 #include <string>
 #include <typeinfo>
 
-namespace spice::hipx
-{
-net_iface * make_net_vogels_abbott( net_params const &, csr_dev &&, hipStream_t );
-net_iface * make_net_brunel( net_params const &, csr_dev &&, hipStream_t );
-net_iface * make_net_brunel_with_plasticity( net_params const &, csr_dev &&, hipStream_t );
-net_iface * make_net_synth( net_params const &, csr_dev &&, hipStream_t );
-net_iface * make_net_brunel_3pop( net_params const &, csr_dev &&, hipStream_t );
-net_iface * make_net_brunel_3pop_stdp( net_params const &, csr_dev &&, hipStream_t );
-batch_iface * make_batch_vogels_abbott( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
-batch_iface * make_batch_brunel( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
-batch_iface * make_batch_brunel_with_plasticity( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
-batch_iface * make_batch_synth( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
-batch_iface * make_batch_brunel_3pop( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
-batch_iface * make_batch_brunel_3pop_stdp( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t );
-} // namespace spice::hipx
 
 using namespace spice;
 using namespace spice::hipx;
@@ -42,6 +26,13 @@ struct spice_batch
 {
 	batch_iface * impl;
 	std::vector<spice_net *> handles; // borrowed per-net handles, released with the batch
+};
+
+struct spice_server
+{
+	std::int64_t chunk;
+	std::unique_ptr<hip_exec> exec;
+	std::unique_ptr<serve::scheduler<hip_exec>> sched; // declared last: stopped first
 };
 
 static thread_local std::string g_err;
@@ -77,69 +68,6 @@ static int guard( F && f )
 	}
 }
 
-static void require( bool c, char const * msg )
-{
-	if( !c ) throw std::invalid_argument( msg );
-}
-
-static util::layout make_layout( uint64_t const * pops, int32_t npops, spice_connection const * conns, int32_t nconns )
-{
-	require( npops >= 0 && nconns >= 0 && ( pops || npops == 0 ) && ( conns || nconns == 0 ), "invalid layout arguments" );
-	std::vector<size_t> p( pops, pops + npops );
-	std::vector<std::tuple<size_t, size_t, float>> c;
-	for( int32_t i = 0; i < nconns; i++ ) c.emplace_back( conns[i].src_pop, conns[i].dst_pop, conns[i].p );
-	return util::layout( p, c );
-}
-
-static std::vector<util::gen::conn> gen_conns( util::layout const & l )
-{
-	std::vector<util::gen::conn> out;
-	for( auto const & c : l.connections() )
-	{
-		util::gen::conn g;
-		g.src_first = std::get<0>( c );
-		g.src_last = std::get<1>( c );
-		g.dst_first = std::get<2>( c );
-		g.dst_last = std::get<3>( c );
-		double const range = double( g.dst_last - g.dst_first );
-		double const p = std::get<4>( c );
-		g.mean = range * p;
-		g.sd = std::sqrt( range * p * ( 1.0 - p ) );
-		out.push_back( g );
-	}
-	return out;
-}
-
-static hipStream_t new_stream()
-{
-	hipStream_t s = nullptr;
-	SPICE_HIP( hipStreamCreateWithFlags( &s, hipStreamNonBlocking ) );
-	return s;
-}
-
-static net_iface * make_net( net_params const & p, csr_dev && g, hipStream_t s )
-{
-	try
-	{
-		switch( p.model )
-		{
-		case SPICE_MODEL_VOGELS_ABBOTT: return make_net_vogels_abbott( p, std::move( g ), s );
-		case SPICE_MODEL_BRUNEL: return make_net_brunel( p, std::move( g ), s );
-		case SPICE_MODEL_BRUNEL_WITH_PLASTICITY: return make_net_brunel_with_plasticity( p, std::move( g ), s );
-		case SPICE_MODEL_SYNTH: return make_net_synth( p, std::move( g ), s );
-		case SPICE_MODEL_BRUNEL_3POP: return make_net_brunel_3pop( p, std::move( g ), s );
-		case SPICE_MODEL_BRUNEL_3POP_STDP: return make_net_brunel_3pop_stdp( p, std::move( g ), s );
-		default: throw std::invalid_argument( "unknown model id" );
-		}
-	}
-	catch( ... )
-	{
-		// the net owns the stream only once constructed
-		(void)hipStreamDestroy( s );
-		throw;
-	}
-}
-
 // Queries and exports of a net borrowed from a batch first wait for the batch's stream.
 static net_iface * impl( spice_net * n )
 {
@@ -154,28 +82,6 @@ static net_iface * stepping_impl( spice_net * n )
 	require( n && n->impl, "null net handle" );
 	require( n->owner == nullptr, "a net that belongs to a batch is stepped through spice_batch_run" );
 	return n->impl;
-}
-
-static batch_iface * make_batch( int model, std::vector<net_params> const & ps, std::vector<csr_dev> && gs, hipStream_t s )
-{
-	try
-	{
-		switch( model )
-		{
-		case SPICE_MODEL_VOGELS_ABBOTT: return make_batch_vogels_abbott( ps, std::move( gs ), s );
-		case SPICE_MODEL_BRUNEL: return make_batch_brunel( ps, std::move( gs ), s );
-		case SPICE_MODEL_BRUNEL_WITH_PLASTICITY: return make_batch_brunel_with_plasticity( ps, std::move( gs ), s );
-		case SPICE_MODEL_SYNTH: return make_batch_synth( ps, std::move( gs ), s );
-		case SPICE_MODEL_BRUNEL_3POP: return make_batch_brunel_3pop( ps, std::move( gs ), s );
-		case SPICE_MODEL_BRUNEL_3POP_STDP: return make_batch_brunel_3pop_stdp( ps, std::move( gs ), s );
-		default: throw std::invalid_argument( "unknown model id" );
-		}
-	}
-	catch( ... )
-	{
-		(void)hipStreamDestroy( s );
-		throw;
-	}
 }
 
 static net_params params_of( int32_t model, spice_net_desc const & d, float dt, uint32_t flags )
@@ -538,6 +444,140 @@ int spice_hip_import_synapse_attr( spice_net * net, int32_t a, void const * in, 
 	return guard( [&] {
 		require( in != nullptr || bytes == 0, "null input" );
 		impl( net )->import_synapse_attr( a, in, bytes, ell != 0 );
+	} );
+}
+
+// ---- serving ----
+static job_spec job_spec_of( spice_job_desc const & j )
+{
+	require( j.dt > 0.0f, "dt must be positive" );
+	require( j.net.delay >= 1, "delay must be >= 1" );
+	require( j.steps >= 1, "a job runs at least one step" );
+	util::layout const l = make_layout( j.net.pops, j.net.npops, j.net.conns, j.net.nconns );
+	require( l.size() >= 1, "a job needs at least one neuron" );
+	job_spec s;
+	s.p = params_of( j.model, j.net, j.dt, j.flags );
+	bool const plastic = dispatch_model( j.model, []( auto m ) { return is_plastic_v<decltype( m )>; } );
+	require( !plastic || j.net.delay <= HISTORY - 1, "plastic models support delays of at most 255 steps" );
+	s.conns = gen_conns( l );
+	s.n = std::int64_t( l.size() );
+	s.width = std::int64_t( l.max_degree() );
+	return s;
+}
+
+static std::uint64_t job_estimate( job_spec const & s, std::int64_t chunk )
+{
+	return estimate_job_bytes( s.p.model, s.n, s.width, s.p.delay, ( s.p.flags & SPICE_FLAG_FIXED_POINT ) != 0, chunk, s.p.dt );
+}
+
+static serve::scheduler<hip_exec> & sched_of( spice_server * sv )
+{
+	require( sv && sv->sched, "null server handle" );
+	return *sv->sched;
+}
+
+int spice_server_create( uint64_t budget_bytes, int64_t chunk_steps, spice_spike_sink sink, void * user, spice_server ** out )
+{
+	return guard( [&] {
+		require( out != nullptr, "null output handle" );
+		require( budget_bytes > 0 && chunk_steps >= 1, "budget and chunk length must be positive" );
+		auto sv = std::make_unique<spice_server>();
+		sv->chunk = chunk_steps;
+		sv->exec = std::make_unique<hip_exec>( sink, user, chunk_steps );
+		sv->sched = std::make_unique<serve::scheduler<hip_exec>>( *sv->exec, budget_bytes, chunk_steps );
+		*out = sv.release();
+	} );
+}
+
+int spice_server_destroy( spice_server * sv )
+{
+	return guard( [&] {
+		if( !sv ) return;
+		sv->sched.reset();
+		sv->exec.reset();
+		delete sv;
+	} );
+}
+
+int spice_server_estimate( spice_job_desc const * job, int64_t chunk_steps, uint64_t * out_bytes )
+{
+	return guard( [&] {
+		require( job && out_bytes && chunk_steps >= 1, "invalid arguments" );
+		*out_bytes = job_estimate( job_spec_of( *job ), chunk_steps );
+	} );
+}
+
+int spice_server_submit( spice_server * sv, spice_job_desc const * job, uint64_t * out_id )
+{
+	return guard( [&] {
+		auto & sc = sched_of( sv );
+		require( job && out_id, "null argument" );
+		std::string const f = sc.fatal();
+		if( !f.empty() ) throw std::runtime_error( "server stopped after an error: " + f );
+		job_spec s = job_spec_of( *job );
+		std::uint64_t const est = job_estimate( s, sv->chunk );
+		*out_id = sc.submit( std::move( s ), job->steps, est );
+	} );
+}
+
+int spice_server_wait( spice_server * sv, uint64_t id, double timeout_s, spice_job_result * out, int32_t * out_finished )
+{
+	return guard( [&] {
+		auto & sc = sched_of( sv );
+		serve::job_record r;
+		bool const fin = sc.wait( id, timeout_s, r );
+		if( out_finished ) *out_finished = fin ? 1 : 0;
+		if( out )
+		{
+			*out = spice_job_result{};
+			out->id = r.id;
+			out->status = r.status;
+			out->steps = r.steps;
+			out->done = r.done;
+			out->synapses = r.synapses;
+			out->device_bytes = r.bytes;
+			out->reserved_bytes = r.est_bytes;
+			out->spikes = r.spikes;
+			out->events = r.events;
+			out->plastic_events = r.plastic_events;
+			out->digest = r.digest;
+			out->t_submit = r.t_submit;
+			out->t_admit = r.t_admit;
+			out->t_done = r.t_done;
+		}
+	} );
+}
+
+int spice_server_cancel( spice_server * sv, uint64_t id )
+{
+	return guard( [&] { sched_of( sv ).cancel( id ); } );
+}
+
+int spice_server_drain( spice_server * sv )
+{
+	return guard( [&] { sched_of( sv ).drain(); } );
+}
+
+int spice_server_get_info( spice_server * sv, spice_server_info * out )
+{
+	return guard( [&] {
+		auto & sc = sched_of( sv );
+		require( out != nullptr, "null output" );
+		serve::server_info const i = sc.info();
+		*out = { i.budget, i.reserved, i.peak_reserved, i.queued, i.resident, i.finished, i.chunks, i.rebuilds, i.steps_launched, i.busy_s, sc.now() };
+	} );
+}
+
+int spice_server_job_error( spice_server * sv, uint64_t id, char * buf, uint64_t cap )
+{
+	return guard( [&] {
+		auto & sc = sched_of( sv );
+		require( buf != nullptr && cap > 0, "null output" );
+		serve::job_record r;
+		sc.wait( id, 0.0, r );
+		std::size_t const n = std::min<std::size_t>( r.error.size(), std::size_t( cap - 1 ) );
+		std::memcpy( buf, r.error.data(), n );
+		buf[n] = 0;
 	} );
 }
 }

@@ -22,6 +22,8 @@
 //   pivots   int32[N * (ntiles+1)]  per row and target tile: offset of the tile's first target
 //   xtr/ytr  f32[N]      neuron-local pre/post traces (trace synapses)
 //   xh/yh    f32[H * N]  their values over the last H steps (ring, slot = step mod H)
+//   slog_ids int32[cap]  spike log (served jobs): ids of steps t0 .. t0+nsteps-1, step by step
+//   slog_off u64[nsteps] offset of step t0+k in slog_ids
 #include <cstdint>
 
 namespace spice::hipx
@@ -51,6 +53,15 @@ enum stat_index
 	ST_FLUSH_SYN,
 	ST_FLUSH_SLOTS,
 	NSTAT = 8
+};
+
+// Spike log window of a net (device memory, set by the host between runs): steps
+// t0 .. t0 + nsteps - 1 append their spike ids to the log, at most cap ids in all.
+struct slog_t
+{
+	std::int64_t t0;
+	std::int64_t nsteps;
+	std::uint64_t cap;
 };
 
 struct dview
@@ -95,5 +106,9 @@ struct dview
 
 	std::int32_t * pivots; // [N][ntiles + 1]: offset in row r of its first target >= t * tile_w
 	std::int32_t ntiles, tile_w;
+
+	slog_t * slog; // nullptr: no spike log
+	std::int32_t * slog_ids;
+	std::uint64_t * slog_off;
 };
 } // namespace spice::hipx

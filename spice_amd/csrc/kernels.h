@@ -490,6 +490,13 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 	int const D1 = v.D + 1;
 	int const slot = int( ( T + 1 ) % D1 ); // ring entry of step T - D
 	std::int32_t const nspk = T < v.D ? 0 : std::int32_t( v.counts[slot] );
+	if( v.slog && tile == 0 && threadIdx.x == 0 )
+	{
+		// spike log: step T's ids start where step T-1's (final by now) end
+		std::int64_t const k = T - v.slog->t0;
+		if( k >= 0 && k < v.slog->nsteps )
+			v.slog_off[k] = k == 0 ? 0ull : v.slog_off[k - 1] + v.counts[( T - 1 ) % D1];
+	}
 	std::int32_t const * spk = v.ring + std::int64_t( slot ) * v.N;
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
@@ -643,8 +650,7 @@ __global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
 	if( nrows > 0 )
 		for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x )
 		{
-			std::int32_t const t0 = tile * v.tile_w;
-				for( std::int32_t c0 = 0; c0 < nrows; c0 += SPK_CHUNK )
+			for( std::int32_t c0 = 0; c0 < nrows; c0 += SPK_CHUNK )
 			{
 				std::int32_t const cnt = ( nrows - c0 ) < SPK_CHUNK ? ( nrows - c0 ) : SPK_CHUNK;
 				std::int32_t len = 0;
@@ -847,7 +853,21 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 		if( t0 ) v.bstats[blk * NSTAT + ST_SPIKES] += t0;
 	}
 	__syncthreads();
-	if( spiked ) v.ring[std::int64_t( wslot ) * v.N + bbase[0] + wcount[w][0] + lanes_below( m )] = std::int32_t( i );
+	if( spiked )
+	{
+		unsigned const o = bbase[0] + wcount[w][0] + lanes_below( m );
+		v.ring[std::int64_t( wslot ) * v.N + o] = std::int32_t( i );
+		if( v.slog )
+		{
+			// the step's offset was written by this step's receive kernel
+			std::int64_t const k = T - v.slog->t0;
+			if( k >= 0 && k < v.slog->nsteps )
+			{
+				std::uint64_t const p = v.slog_off[k] + o;
+				if( p < v.slog->cap ) v.slog_ids[p] = std::int32_t( i );
+			}
+		}
+	}
 	if constexpr( plastic )
 	{
 		if( flush )
@@ -873,8 +893,10 @@ __global__ __launch_bounds__( 256 ) void k_batch_update( dview const * views, st
 	update_block<Model, FIXED>( views[block_net[blockIdx.x]], block_local[blockIdx.x] );
 }
 
+// Row-major flush of the rows whose history window is full: work item = (row, part); waves
+// of workgroups blk, blk + nblk, ... (of this net) take items round robin.
 template <typename Model, bool FIXED>
-__global__ __launch_bounds__( 256 ) void k_flush( dview v )
+__device__ inline void flush_block( dview const & v, std::int64_t blk, std::int64_t nblk )
 {
 	using S = typename Model::synapse;
 	constexpr int U = 4; // slots per lane per iteration: independent loads in flight
@@ -883,8 +905,8 @@ __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
 	int const lane = lane_id();
-	std::int64_t const wave0 = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
-	std::int64_t const nwaves = std::int64_t( gridDim.x ) * ( blockDim.x >> 6 );
+	std::int64_t const wave0 = uniform64( ( blk * blockDim.x + threadIdx.x ) >> 6 );
+	std::int64_t const nwaves = nblk * ( blockDim.x >> 6 );
 	unsigned long long touched = 0, slots = 0, rows = 0;
 	for( std::int64_t it = wave0; it < nitems; it += nwaves )
 	{
@@ -940,7 +962,35 @@ __global__ __launch_bounds__( 256 ) void k_flush( dview v )
 			if( p == 0 ) rows++;
 		}
 	}
-	block_add_stats<3>( v, blockIdx.x, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	block_add_stats<3>( v, blk, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+}
+
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( 256 ) void k_flush( dview v )
+{
+	flush_block<Model, FIXED>( v, blockIdx.x, gridDim.x );
+}
+
+// Batched flush: workgroup b is flush block block_local[b] of the block_count[b] blocks of
+// net block_net[b].
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( 256 ) void k_batch_flush(
+    dview const * views, std::int32_t const * block_net, std::int32_t const * block_local, std::int32_t const * block_count )
+{
+	std::int32_t const b = blockIdx.x;
+	flush_block<Model, FIXED>( views[block_net[b]], block_local[b], block_count[b] );
+}
+
+// Opens the spike-log window of every net of a batch: the next nsteps steps, starting at the
+// net's current step (read from its clock slot in `views`).
+static __global__ __launch_bounds__( 256 ) void k_batch_log_window( dview const * views, std::int32_t n, std::int64_t nsteps )
+{
+	std::int32_t const k = std::int32_t( blockIdx.x * blockDim.x + threadIdx.x );
+	if( k >= n ) return;
+	dview const & v = views[k];
+	if( !v.slog ) return;
+	v.slog->t0 = v.ctl->step;
+	v.slog->nsteps = nsteps;
 }
 
 // Brings every row with plastic synapses up to date through step `now` (= last completed

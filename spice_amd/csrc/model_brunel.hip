@@ -15,4 +15,9 @@ batch_iface * make_batch_brunel( std::vector<net_params> const & ps, std::vector
 {
 	return make_batch<::spice::brunel>( ps, std::move( gs ), s );
 }
+
+batch_iface * make_group_brunel( std::vector<net_iface *> const & nets, hipStream_t s )
+{
+	return make_group<::spice::brunel>( nets, s );
+}
 } // namespace spice::hipx

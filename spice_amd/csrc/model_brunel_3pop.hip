@@ -15,4 +15,9 @@ batch_iface * make_batch_brunel_3pop( std::vector<net_params> const & ps, std::v
 {
 	return make_batch<::spice::brunel_3pop>( ps, std::move( gs ), s );
 }
+
+batch_iface * make_group_brunel_3pop( std::vector<net_iface *> const & nets, hipStream_t s )
+{
+	return make_group<::spice::brunel_3pop>( nets, s );
+}
 } // namespace spice::hipx

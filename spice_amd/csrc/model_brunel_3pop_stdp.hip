@@ -11,8 +11,13 @@ net_iface * make_net_brunel_3pop_stdp( net_params const & p, csr_dev && g, hipSt
 	return new hip_net<::spice::brunel_3pop_stdp>( p, std::move( g ), s );
 }
 
-batch_iface * make_batch_brunel_3pop_stdp( std::vector<net_params> const &, std::vector<csr_dev> &&, hipStream_t )
+batch_iface * make_batch_brunel_3pop_stdp( std::vector<net_params> const & ps, std::vector<csr_dev> && gs, hipStream_t s )
 {
-	throw std::invalid_argument( "batches hold static (non-plastic) models only" );
+	return make_batch<::spice::brunel_3pop_stdp>( ps, std::move( gs ), s );
+}
+
+batch_iface * make_group_brunel_3pop_stdp( std::vector<net_iface *> const & nets, hipStream_t s )
+{
+	return make_group<::spice::brunel_3pop_stdp>( nets, s );
 }
 } // namespace spice::hipx

@@ -15,4 +15,9 @@ batch_iface * make_batch_synth( std::vector<net_params> const & ps, std::vector<
 {
 	return make_batch<::spice::synth>( ps, std::move( gs ), s );
 }
+
+batch_iface * make_group_synth( std::vector<net_iface *> const & nets, hipStream_t s )
+{
+	return make_group<::spice::synth>( nets, s );
+}
 } // namespace spice::hipx

@@ -15,4 +15,9 @@ batch_iface * make_batch_vogels_abbott( std::vector<net_params> const & ps, std:
 {
 	return make_batch<::spice::vogels_abbott>( ps, std::move( gs ), s );
 }
+
+batch_iface * make_group_vogels_abbott( std::vector<net_iface *> const & nets, hipStream_t s )
+{
+	return make_group<::spice::vogels_abbott>( nets, s );
+}
 } // namespace spice::hipx

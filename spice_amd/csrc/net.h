@@ -57,6 +57,18 @@ struct net_iface
 	virtual void import_neuron_attr( int a, void const * in, std::uint64_t bytes ) = 0;
 	virtual void export_synapse_attr( int a, void * out, std::uint64_t bytes, bool ell ) = 0;
 	virtual void import_synapse_attr( int a, void const * in, std::uint64_t bytes, bool ell ) = 0;
+
+	// Spike log (served jobs). enable_spike_log allocates a log of `cap` ids for windows of at
+	// most `max_steps` steps (before the first run); begin_spike_log opens a window over the
+	// next `nsteps` steps; once the net has run exactly those steps, read_spike_log copies the
+	// window out: off[0..nsteps] (off[k] = first id of step t0+k, off[nsteps] = total) and the
+	// ids, and returns the total (> cap: the log overflowed, ids past cap were dropped). No step
+	// of the net may be in flight during begin/read.
+	virtual void enable_spike_log( std::uint64_t cap, std::int64_t max_steps ) = 0;
+	virtual void begin_spike_log( std::int64_t nsteps ) = 0;
+	virtual std::uint64_t read_spike_log( std::uint64_t * off, std::int32_t * ids, std::uint64_t cap ) = 0;
+	virtual std::uint64_t device_bytes() const = 0;
+	virtual std::int64_t host_step() const = 0;
 };
 
 inline int device_cu_count()
@@ -343,11 +355,85 @@ public:
 		if( _g.nnz ) SPICE_HIP( hipMemcpy( _sa[a].data(), csr.data(), csr.size(), hipMemcpyHostToDevice ) );
 	}
 
+	// ---- spike log ----
+	void enable_spike_log( std::uint64_t cap, std::int64_t max_steps ) override
+	{
+		if( _exec || _slog_ctl.size() ) throw std::logic_error( "the spike log is enabled once, before the first run" );
+		if( max_steps < 1 ) throw std::invalid_argument( "spike log window must hold at least one step" );
+		_slog_ctl.resize( 1 );
+		_slog_ids.resize( std::max<std::uint64_t>( cap, 1 ) );
+		_slog_off.resize( std::size_t( max_steps ) );
+		_slog_cap = cap;
+		_slog_max = max_steps;
+		slog_t const w{ 0, 0, cap }; // empty window
+		SPICE_HIP( hipMemcpy( _slog_ctl.data(), &w, sizeof( w ), hipMemcpyHostToDevice ) );
+		for( dview * v : { &_v, &_vs[0], &_vs[1] } )
+		{
+			v->slog = _slog_ctl.data();
+			v->slog_ids = _slog_ids.data();
+			v->slog_off = _slog_off.data();
+		}
+	}
+
+	void begin_spike_log( std::int64_t nsteps ) override
+	{
+		if( !_slog_ctl.size() ) throw std::logic_error( "spike log not enabled" );
+		if( nsteps < 0 || nsteps > _slog_max ) throw std::invalid_argument( "spike log window too long" );
+		slog_t const w{ _step, nsteps, _slog_cap };
+		SPICE_HIP( hipMemcpy( _slog_ctl.data(), &w, sizeof( w ), hipMemcpyHostToDevice ) );
+		note_spike_log( nsteps );
+	}
+
+	// Host bookkeeping of a window whose device side a batch opened (k_batch_log_window).
+	void note_spike_log( std::int64_t nsteps )
+	{
+		if( !_slog_ctl.size() ) return;
+		if( nsteps < 0 || nsteps > _slog_max ) throw std::invalid_argument( "spike log window too long" );
+		_slog_t0 = _step;
+		_slog_n = nsteps;
+	}
+
+	std::uint64_t read_spike_log( std::uint64_t * off, std::int32_t * ids, std::uint64_t cap ) override
+	{
+		if( !_slog_ctl.size() ) throw std::logic_error( "spike log not enabled" );
+		if( _step != _slog_t0 + _slog_n ) throw std::logic_error( "spike log window not completed" );
+		off[0] = 0;
+		if( _slog_n == 0 ) return 0;
+		SPICE_HIP( hipMemcpy( off, _slog_off.data(), std::size_t( _slog_n ) * 8, hipMemcpyDeviceToHost ) );
+		std::uint32_t last = 0;
+		int const slot = int( ( _step - 1 ) % ( _p.delay + 1 ) );
+		SPICE_HIP( hipMemcpy( &last, _counts.data() + slot, 4, hipMemcpyDeviceToHost ) );
+		std::uint64_t const total = off[_slog_n - 1] + last;
+		off[_slog_n] = total;
+		std::uint64_t const n = std::min( { total, _slog_cap, cap } );
+		if( n ) SPICE_HIP( hipMemcpy( ids, _slog_ids.data(), std::size_t( n ) * 4, hipMemcpyDeviceToHost ) );
+		return total;
+	}
+
+	std::uint64_t device_bytes() const override
+	{
+		std::uint64_t b = _g.row_ptr.bytes() + _g.col.bytes();
+		for( int a = 0; a < MAXA; a++ ) b += _na[a].bytes() + _acc[a].bytes() + _sa[a].bytes();
+		b += _ring.bytes() + _counts.bytes() + _hring.bytes() + _last_spk.bytes() + _synced.bytes() + _pend.bytes() +
+		     _fl_id.bytes() + _fl_synced.bytes() + _fl_pend.bytes() + _hasp.bytes() + _pl_lo.bytes() + _pl_hi.bytes() +
+		     _fl_count.bytes() + _xtr.bytes() + _ytr.bytes() + _xh.bytes() + _yh.bytes() + _ctl.bytes() + _bstats.bytes() +
+		     _pivots.bytes() + _slog_ctl.bytes() + _slog_ids.bytes() + _slog_off.bytes();
+		return b;
+	}
+
 	// ---- used by hip_batch (batched stepping of many nets) ----
 	dview const & view( int parity ) const { return _vs[parity & 1]; }
 	int num_tiles() const { return _v.ntiles; }
 	int num_update_blocks() const { return _grid_update; }
-	std::int64_t host_step() const { return _step; }
+	// flush workgroups of this net inside a batch: about two per expected work item wave
+	// (rows reach their flush about every FLUSH_AGE steps), at most the net's own grid
+	int num_batch_flush_blocks() const
+	{
+		std::int64_t const items = ( _g.n / FLUSH_AGE + 1 ) * _v.fparts;
+		return int( std::clamp<std::int64_t>( ( items + 1 ) / 2, 1, _grid_flush ) );
+	}
+	bool fixed() const { return ( _p.flags & SPICE_FLAG_FIXED_POINT ) != 0; }
+	std::int64_t host_step() const override { return _step; }
 	void advance_host( std::int64_t n )
 	{
 		for( std::int64_t k = 0; k < n; k++ )
@@ -384,7 +470,11 @@ private:
 	std::int64_t _step = 0;
 	util::kahan_sum<float> _clock;
 
-	bool fixed() const { return ( _p.flags & SPICE_FLAG_FIXED_POINT ) != 0; }
+	dbuf<slog_t> _slog_ctl;
+	dbuf<std::int32_t> _slog_ids;
+	dbuf<std::uint64_t> _slog_off;
+	std::uint64_t _slog_cap = 0;
+	std::int64_t _slog_max = 0, _slog_t0 = 0, _slog_n = 0;
 
 	template <typename Decl>
 	static std::size_t attr_bytes( int a )
@@ -505,7 +595,7 @@ private:
 		_pivots.resize( std::size_t( n ) * ( v.ntiles + 1 ) );
 		v.pivots = _pivots.data();
 		_grid_tiles = std::min( v.ntiles, 16 * cus );
-		int const max_blocks = std::max( { _grid_update, _grid_receive, _grid_flush, _grid_tiles } );
+		int const max_blocks = std::max( { _grid_update, _grid_receive, _grid_flush, v.ntiles } ); // receive slots are tiles
 		_bstats.resize( std::size_t( max_blocks ) * NSTAT );
 		SPICE_HIP( hipMemsetAsync( _bstats.data(), 0, _bstats.bytes(), _s ) );
 		v.bstats = _bstats.data();

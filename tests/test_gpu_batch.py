@@ -47,6 +47,37 @@ def test_batch_equals_networks_alone(gpu, model):
     b.close()
 
 
+@pytest.mark.parametrize("model", ["brunel_3pop_stdp", "brunel_with_plasticity"])
+def test_plastic_batch_equals_networks_alone(gpu, model):
+    """Plastic nets in one batch (batched receive, update and lazy-STDP flush launches): spikes,
+    neuron state, synaptic weights and counters equal each net run alone, over enough steps
+    for every row to be flushed at least once (window 255 steps)."""
+    rng = np.random.default_rng(11)
+    nets = []
+    for k in range(5):
+        n = int(rng.integers(400, 2500))
+        lay = three_pop_layout(n) if model == "brunel_3pop_stdp" else ([n // 2, n - n // 2], [(0, 1, 0.1), (1, 1, 0.1)])
+        nets.append((Layout(*lay), int(rng.choice([1, 4, 15])), 500 + k, None))
+    steps = 600
+    b = Batch(model, nets, 1e-4, fixed_point=True)
+    b.run(steps)
+    b.synchronize()
+    for k, (lay, delay, seed, _) in enumerate(nets):
+        alone = Snn(model, lay, 1e-4, delay, seed=seed, fixed_point=True)
+        alone.run(steps)
+        bn = b.net(k)
+        assert_state_equal(bn.neurons(), alone.neurons(), f"net {k}")
+        sa, sb = alone.stats(), bn.stats()
+        for key in ("spikes", "events", "plastic_events", "flush_rows", "flush_synapses"):
+            assert sa[key] == sb[key], (k, key, sa, sb)
+        wa, wb = alone.synapses(ell=False), bn.synapses(ell=False)
+        assert wa.keys() == wb.keys()
+        for key in wa:
+            np.testing.assert_array_equal(wa[key], wb[key], err_msg=f"net {k} synapse attribute {key}")
+        alone.close()
+    b.close()
+
+
 def test_batch_net_parameters_match_checker(gpu):
     """A per-network Poisson rate (snn_info::param[0]) reaches the model identically on the GPU
     (inside a batch) and in the CPU checker."""
@@ -69,8 +100,6 @@ def test_borrowed_nets_cannot_be_stepped_or_destroyed(gpu):
         n.step()
     with pytest.raises(ValueError):
         n.run(5)
-    with pytest.raises(ValueError):
-        Batch("brunel_3pop_stdp", [(Layout(*three_pop_layout(500)), 1, 1)], 1e-4)
     b.run(10)
     assert n.clock()[0] == 10
     b.close()
