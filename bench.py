@@ -43,6 +43,7 @@ WORKLOADS = {
     "brunel_stdp_1e9": ("brunel_3pop_stdp", 100_000, 0.1, 15),
     "brunel_1e9": ("brunel_3pop", 100_000, 0.1, 15),
     "vogels_1e8": ("vogels_abbott", 70_710, 0.02, 1),
+    "brunel_1e10": ("brunel_3pop", 316_228, 0.1, 15),
 }
 
 
@@ -55,8 +56,9 @@ def algorithmic_bytes(model, n, st, steps):
                plastic synapse changed, the write-back of its state (4 for neuron-trace
                synapses, 12 + 8 re-read for per-synapse-trace synapses);
       update : neuron state read + write, accumulator read + clear, plus for plastic models
-               history (16), row bookkeeping (9) and, for neuron-trace synapses, traces (16)
-               and trace rings (8) per neuron; 4 per emitted spike;
+               history word read + write (16), row bookkeeping (9)
+               and, for neuron-trace synapses, traces (16) and trace rings (8) per neuron;
+               4 per emitted spike;
       flush  : target ids of every scanned slot of the flushed rows (4) + per changed synapse
                its state read + write (8 neuron-trace, 24 per-synapse-trace).
     Target-neuron accumulation is counted as atomics, not bytes."""
@@ -282,6 +284,7 @@ def main():
     ap.add_argument("--batch-nets", type=int, default=512, help="nets of the batched sweep (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-seq-stride", type=int, default=8)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--serve-jobs", type=int, default=64, help="jobs of the served stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, jobs per second")
     ap.add_argument("--serve-budget", type=float, default=250e9, help="HBM admission budget, bytes")
@@ -306,6 +309,12 @@ def main():
     setup_s = time.perf_counter() - t0
     E = net.num_edges()
     print(f"[bench] {args.workload}: N={n} E={E} setup {setup_s:.3f}s", file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        cfg, us = net.autotune(warmup=500, steps=100)
+        tune = {"launch": cfg, "predicted_us_per_step": round(us, 3), "tune_s": round(time.perf_counter() - t0, 3)}
+        print(f"[bench] autotuned {cfg} ({us:.1f} us/step) in {tune['tune_s']}s", file=sys.stderr, flush=True)
 
     net.run(args.warmup)
     net.synchronize()
@@ -343,6 +352,7 @@ def main():
                          f"state from init; dense per-step plasticity as in cpu::snn), "
                          f"{s_per_step:.2f} s/step, wall/bio extrapolated linearly; init {t_init:.1f}s excluded"}
 
+    net_launch = net.launch()
     net.close()
     batched = run_batched(args.batch_nets, args.batch_steps, args.batch_seq_stride) if args.batch_nets > 0 else None
     served = (run_serving(args.serve_jobs, args.serve_rate, args.serve_budget, args.serve_chunk,
@@ -367,6 +377,8 @@ def main():
         "spikes_per_step": round(st["spikes"] / args.steps, 2),
         "events_per_step": round(st["events"] / args.steps, 1),
         "setup_s": round(setup_s, 4),
+        "launch": net_launch,
+        "autotune": tune,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

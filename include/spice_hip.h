@@ -240,6 +240,27 @@ int spice_batch_size( spice_batch * batch, int32_t * out_nnets );
  * setting its clock or destroying it is rejected. */
 int spice_batch_net( spice_batch * batch, int32_t i, spice_net ** out_net );
 
+/* ---- launch shapes and the autotuner --------------------------------------------------- */
+/* No counterpart in the reference (its launch shapes are compile-time constants). tile_w:
+ * targets per spike-delivery tile (multiple of 64, 64..1024); flush_slots: target slots per
+ * lazy-plasticity flush work item (multiple of 64); flush_grid: flush workgroups. A zero
+ * field selects the default. Launch shapes never change results. */
+typedef struct spice_launch
+{
+	int32_t tile_w;
+	int32_t flush_slots;
+	int32_t flush_grid;
+	int32_t reserved;
+} spice_launch;
+
+int spice_hip_get_launch( spice_net * net, spice_launch * out );
+int spice_hip_set_launch( spice_net * net, const spice_launch * in );
+/* Times candidate launch shapes on a copy of the network (warmup steps first, then `steps`
+ * steps per candidate, per-kernel HIP-event times), applies the fastest to the network and
+ * returns it with its step time (us). The network itself does not advance. Results are
+ * cached per (model, neurons, edges, delay, flags) for the life of the process. */
+int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_launch * out, double * out_us_per_step );
+
 /* ---- serving: a request loop over device-resident simulation jobs ------------------------ */
 /* No counterpart in the reference. A server admits submitted jobs in arrival order while the
  * sum of their device-memory reservations fits its budget, keeps each admitted job's graph

@@ -47,6 +47,11 @@ class Stats(ctypes.Structure):
                 ("steps", "spikes", "events", "plastic_events", "flush_rows", "flush_synapses", "flush_slots")]
 
 
+class Launch(ctypes.Structure):
+    _fields_ = [("tile_w", ctypes.c_int32), ("flush_slots", ctypes.c_int32), ("flush_grid", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class JobDesc(ctypes.Structure):
     _fields_ = [("model", ctypes.c_int32), ("net", NetDesc), ("dt", ctypes.c_float), ("flags", ctypes.c_uint32),
                 ("steps", ctypes.c_int64)]
@@ -92,6 +97,9 @@ SIGNATURES = {
     "spice_batch_synchronize": (ctypes.c_int, [_P]),
     "spice_batch_size": (ctypes.c_int, [_P, _PI32]),
     "spice_batch_net": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_P)]),
+    "spice_hip_get_launch": (ctypes.c_int, [_P, ctypes.POINTER(Launch)]),
+    "spice_hip_set_launch": (ctypes.c_int, [_P, ctypes.POINTER(Launch)]),
+    "spice_hip_autotune": (ctypes.c_int, [_P, _I64, _I64, ctypes.POINTER(Launch), ctypes.POINTER(ctypes.c_double)]),
     "spice_server_create": (ctypes.c_int, [_U64, _I64, SPIKE_SINK, _P, ctypes.POINTER(_P)]),
     "spice_server_destroy": (ctypes.c_int, [_P]),
     "spice_server_estimate": (ctypes.c_int, [ctypes.POINTER(JobDesc), _I64, _PU64]),

@@ -7,6 +7,10 @@
 #include <spice_hip.h>
 
 #include <climits>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstring>
 #include <new>
 #include <string>
@@ -578,6 +582,114 @@ int spice_server_job_error( spice_server * sv, uint64_t id, char * buf, uint64_t
 		std::size_t const n = std::min<std::size_t>( r.error.size(), std::size_t( cap - 1 ) );
 		std::memcpy( buf, r.error.data(), n );
 		buf[n] = 0;
+	} );
+}
+
+// ---- launch shapes and the autotuner ----
+int spice_hip_get_launch( spice_net * net, spice_launch * out )
+{
+	return guard( [&] {
+		require( out != nullptr, "null output" );
+		launch_cfg const c = impl( net )->get_launch();
+		*out = { c.tile_w, c.flush_slots, c.flush_grid, 0 };
+	} );
+}
+
+int spice_hip_set_launch( spice_net * net, spice_launch const * in )
+{
+	return guard( [&] {
+		require( in != nullptr, "null input" );
+		stepping_impl( net )->set_launch( { in->tile_w, in->flush_slots, in->flush_grid } );
+	} );
+}
+
+namespace
+{
+using tune_key = std::tuple<int, std::int64_t, std::int64_t, int, std::uint32_t>;
+std::mutex g_tune_mutex;
+std::map<tune_key, std::pair<launch_cfg, double>> g_tune_cache;
+} // namespace
+
+int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_launch * out, double * out_us )
+{
+	return guard( [&] {
+		net_iface * n = stepping_impl( net );
+		require( warmup >= 0 && steps >= 1, "warmup >= 0 and steps >= 1 required" );
+		net_params const & p = n->params();
+		tune_key const key{ p.model, n->num_neurons(), n->num_edges(), p.delay, p.flags };
+		std::pair<launch_cfg, double> best;
+		bool hit = false;
+		{
+			std::lock_guard<std::mutex> g( g_tune_mutex );
+			auto it = g_tune_cache.find( key );
+			if( it != g_tune_cache.end() )
+			{
+				best = it->second;
+				hit = true;
+			}
+		}
+		if( !hit )
+		{
+			std::unique_ptr<net_iface> c( n->clone() );
+			c->run( warmup );
+			c->synchronize();
+			// per-step time (ms) of kernel k (0 receive, 1 update, 2 flush) under cfg
+			auto time_of = [&]( launch_cfg cfg, int k ) {
+				c->set_launch( cfg );
+				double ms[3] = { 0, 0, 0 };
+				c->run_profiled( steps, ms );
+				return ms[k] / double( steps );
+			};
+			launch_cfg cfg = c->get_launch();
+			double const inf = std::numeric_limits<double>::infinity();
+			double t_recv = inf, t_flush = 0.0;
+			std::int64_t const N = n->num_neurons();
+			for( int tw : { 128, 256, 384, 512, 768, 1024 } )
+			{
+				std::int64_t const tiles = ( N + tw - 1 ) / tw;
+				if( N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) ) continue; // pivots above 4 GB
+				launch_cfg x = cfg;
+				x.tile_w = tw;
+				double const t = time_of( x, 0 );
+				if( t < t_recv )
+				{
+					t_recv = t;
+					cfg.tile_w = tw;
+				}
+			}
+			if( n->is_plastic() )
+			{
+				int cus = 0;
+				{
+					int dev = 0;
+					SPICE_HIP( hipGetDevice( &dev ) );
+					SPICE_HIP( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
+				}
+				t_flush = inf;
+				launch_cfg const base = cfg;
+				for( int fs : { 64, 128, 256, 512, 1024 } )
+					for( int fg : { 2, 4, 8, 16 } )
+					{
+						launch_cfg x = base;
+						x.flush_slots = fs;
+						x.flush_grid = fg * cus;
+						double const t = time_of( x, 2 );
+						if( t < t_flush )
+						{
+							t_flush = t;
+							cfg.flush_slots = fs;
+							cfg.flush_grid = x.flush_grid;
+						}
+					}
+			}
+			double const t_upd = time_of( cfg, 1 );
+			best = { cfg, ( t_recv + t_upd + t_flush ) * 1e3 };
+			std::lock_guard<std::mutex> g( g_tune_mutex );
+			g_tune_cache[key] = best;
+		}
+		n->set_launch( best.first );
+		if( out ) *out = { best.first.tile_w, best.first.flush_slots, best.first.flush_grid, 0 };
+		if( out_us ) *out_us = best.second;
 	} );
 }
 }

@@ -68,7 +68,6 @@ struct dview
 {
 	std::int32_t N, D, width, shift;
 	float DT;
-	std::int32_t parts;  // work items per delivered spike row (receive)
 	std::int32_t fparts; // work items per flushed row
 	std::uint32_t flags;
 	std::uint64_t seed;

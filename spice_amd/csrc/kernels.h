@@ -34,7 +34,12 @@ namespace spice::hipx
 {
 constexpr int HISTORY = 256;            // spike-history window (steps) of lazy plasticity
 constexpr int HWORDS = HISTORY / 64;     // 64-bit words per neuron in the history ring
-constexpr int FLUSH_AGE = HISTORY - 1;   // rows are brought up to date at least this often
+// Rows are brought up to date at least every flush_age_v<Synapse> steps. Trace synapses replay
+// events only (bit-exact with per-step updates for any window), so they use the whole
+// history; the generic contract merges an event-free stretch into one update(n) whose decay
+// pow(d, n) differs from n per-step products by up to ~n/2 ulp, so its stretches stay short.
+template <typename S>
+constexpr int flush_age_v = has_neuron_traces_v<S> ? HISTORY - 1 : 255;
 
 template <typename T>
 __device__ inline T ld( void const * base, std::int64_t i )
@@ -216,15 +221,24 @@ __device__ inline bool spiked_at( dview const & v, std::int32_t j, std::int32_t 
 	return ( v.hring[w * v.N + j] >> ( e & 63 ) ) & 1ull;
 }
 
-// f(e) for every step e in [first, last] at which neuron j spiked, in increasing order
-// (last - first < HISTORY). All history words of j are loaded up front (independent loads).
-template <typename F>
-__device__ inline void for_each_spike( dview const & v, std::int32_t j, std::int32_t first, std::int32_t last, F && f )
+// History words of neuron j (independent loads, issued together).
+struct hwords
 {
-	if( first > last ) return;
-	unsigned long long words[HWORDS];
+	unsigned long long w[HWORDS];
+};
+__device__ inline hwords load_hwords( dview const & v, std::int32_t j )
+{
+	hwords h;
 #pragma unroll
-	for( int w = 0; w < HWORDS; w++ ) words[w] = v.hring[std::int64_t( w ) * v.N + j];
+	for( int w = 0; w < HWORDS; w++ ) h.w[w] = v.hring[std::int64_t( w ) * v.N + j];
+	return h;
+}
+
+// f(e) for every step e in [first, last] at which the neuron with history words h spiked,
+// in increasing order (last - first < HISTORY).
+template <typename F>
+__device__ inline void for_each_spike( hwords const & h, std::int32_t first, std::int32_t last, F && f )
+{
 	for( std::int32_t e = first; e <= last; )
 	{
 		int const idx = e & ( HISTORY - 1 );
@@ -233,7 +247,7 @@ __device__ inline void for_each_spike( dview const & v, std::int32_t j, std::int
 		unsigned long long word = 0;
 #pragma unroll
 		for( int w = 0; w < HWORDS; w++ )
-			if( w == ( idx >> 6 ) ) word = words[w];
+			if( w == ( idx >> 6 ) ) word = h.w[w];
 		unsigned long long bits = word >> b0;
 		if( cnt < 64 ) bits &= ( 1ull << cnt ) - 1ull;
 		while( bits )
@@ -244,6 +258,13 @@ __device__ inline void for_each_spike( dview const & v, std::int32_t j, std::int
 		}
 		e += cnt;
 	}
+}
+
+template <typename F>
+__device__ inline void for_each_spike( dview const & v, std::int32_t j, std::int32_t first, std::int32_t last, F && f )
+{
+	if( first > last ) return;
+	for_each_spike( load_hwords( v, j ), first, last, f );
 }
 
 // Applies, in time order, the plasticity of every step in (synced, now] to one synapse
@@ -297,6 +318,66 @@ __device__ inline bool replay_traced(
 		Syn::on_event( syn, false, true, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
 	} );
 	return true;
+}
+
+// Bit e (a step within the window) of a neuron's history words.
+__device__ inline bool hbit( hwords const & h, std::int32_t e )
+{
+	int const idx = e & ( HISTORY - 1 );
+	unsigned long long word = 0;
+#pragma unroll
+	for( int w = 0; w < HWORDS; w++ )
+		if( w == ( idx >> 6 ) ) word = h.w[w];
+	return ( word >> ( idx & 63 ) ) & 1ull;
+}
+
+// Everything replay_traced reads that does not depend on the event list, loaded ahead for
+// several synapses at once: the target's latest spike and history words (if it spiked after
+// synced) and, for a pending pre event, the target's post trace at that step (the source's
+// pre trace at that step, xf, is one value per row).
+struct traced_pre
+{
+	hwords h;
+	float yf;
+	std::int32_t ls;
+};
+
+__device__ inline traced_pre load_traced( dview const & v, std::int32_t j, std::int32_t ls, std::int32_t synced, std::int32_t pend )
+{
+	traced_pre t;
+	t.ls = ls;
+	if( ls > synced )
+		t.h = load_hwords( v, j );
+	else
+	{
+#pragma unroll
+		for( int w = 0; w < HWORDS; w++ ) t.h.w[w] = 0ull;
+	}
+	t.yf = pend ? v.yh[std::int64_t( ( synced + 1 ) & ( HISTORY - 1 ) ) * v.N + j] : 0.0f;
+	return t;
+}
+
+// replay_traced over preloaded data (same event sequence and arithmetic): only the traces at
+// the target's post spikes in the window are loaded here.
+template <typename Syn, typename Iter, typename Bak>
+__device__ inline void replay_traced_pre(
+    Iter syn, dview const & v, std::int32_t s, std::int32_t j, std::int32_t now, std::int32_t synced, std::int32_t pend, float xf,
+    traced_pre const & t, snn_info info, Bak & bak )
+{
+	std::int64_t const N = v.N;
+	std::int32_t first = synced + 1;
+	if( pend )
+	{
+		bool const post = t.ls >= first && hbit( t.h, first );
+		Syn::on_event( syn, true, post, xf, t.yf, info, bak );
+		first++;
+	}
+	std::int32_t const last = t.ls < now ? t.ls : now;
+	if( first <= last )
+		for_each_spike( t.h, first, last, [&]( std::int32_t e ) {
+			std::int64_t const slot = e & ( HISTORY - 1 );
+			Syn::on_event( syn, false, true, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
+		} );
 }
 
 // Whether synapse src -> j has an event in (synced, now] (trace synapses only need those).
@@ -484,6 +565,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 	__shared__ std::int32_t seg_src[SPK_CHUNK];
 	__shared__ std::int32_t seg_synced[plastic ? SPK_CHUNK : 1];
 	__shared__ std::int32_t seg_pend[plastic ? SPK_CHUNK : 1];
+	__shared__ float seg_xf[plastic && has_neuron_traces_v<S> ? SPK_CHUNK : 1];
 	__shared__ std::int32_t wsum[TBS / 64];
 
 	std::int64_t const T = v.ctl->step;
@@ -522,8 +604,11 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 				len = b - a;
 				if constexpr( plastic )
 				{
-					seg_synced[tid] = v.synced[s];
-					seg_pend[tid] = v.pend[s];
+					std::int32_t const sy = v.synced[s], pd = v.pend[s];
+					seg_synced[tid] = sy;
+					seg_pend[tid] = pd;
+					if constexpr( has_neuron_traces_v<S> ) // the source's pre trace at its pending event
+						seg_xf[tid] = pd ? v.xh[std::int64_t( ( sy + 1 ) & ( HISTORY - 1 ) ) * v.N + s] : 0.0f;
 				}
 			}
 			// exclusive block scan of the segment lengths
@@ -548,6 +633,49 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 					e[u] = seg_beg[a] + ( f - seg_pre[a] );
 					j[u] = ok[u] ? v.col[e[u]] : 0;
 				}
+				if constexpr( plastic && has_neuron_traces_v<S> )
+				{
+					// trace synapses: load every synapse's state and replay inputs first (independent
+					// loads for all u), then replay the events and deliver
+					typename S::tuple_t sy[U];
+					traced_pre tp[U];
+					bool ev[U];
+					std::int32_t ls[U];
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+						if( ok[u] ) load_tuple<S>( sy[u], v.sa, e[u] );
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						ev[u] = ok[u] && S::plastic( seg_src[kk[u]], j[u], info );
+						ls[u] = ev[u] ? v.last_spk[j[u]] : -1;
+					}
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						std::int32_t const synced = seg_synced[kk[u]], pend = seg_pend[kk[u]];
+						ev[u] = ev[u] && std::int32_t( T - 1 ) > synced && ( pend || ls[u] > synced );
+						if( ev[u] ) tp[u] = load_traced( v, j[u], ls[u], synced, pend );
+					}
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						if( !ok[u] ) continue;
+						std::int32_t const s = seg_src[kk[u]];
+						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
+						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc };
+						reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
+						if( ev[u] )
+						{
+							replay_traced_pre<S>( itr, v, s, j[u], std::int32_t( T - 1 ), seg_synced[kk[u]], seg_pend[kk[u]], seg_xf[kk[u]], tp[u], info, bak );
+							store_tuple<S>( sy[u], v.sa, e[u] );
+							pev++;
+						}
+						N::receive( s, dst, itr, info, bak );
+					}
+				}
+				else
+				{
 #pragma unroll
 				for( int u = 0; u < U; u++ )
 				{
@@ -563,24 +691,15 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 						if( S::plastic( s, j[u], info ) )
 						{
 							std::int32_t const synced = seg_synced[kk[u]], pend = seg_pend[kk[u]];
-							bool changed;
-							if constexpr( has_neuron_traces_v<S> )
-								changed = replay_traced<S>( itr, v, s, j[u], std::int32_t( T - 1 ), synced, pend, info, bak );
-							else
-							{
-								replay<S>( itr, v, j[u], std::int32_t( T - 1 ), synced, pend, dt, info, bak );
-								changed = true;
-							}
-							if( changed )
-							{
-								store_tuple<S>( sy, v.sa, e[u] );
-								pev++;
-							}
+							replay<S>( itr, v, j[u], std::int32_t( T - 1 ), synced, pend, dt, info, bak );
+							store_tuple<S>( sy, v.sa, e[u] );
+							pev++;
 						}
 						N::receive( s, dst, itr, info, bak );
 					}
 					else
 						N::receive( s, dst, no_syn_iter{}, info, bak );
+				}
 				}
 			}
 			if( tid == 0 ) ev += std::uint64_t( total );
@@ -787,11 +906,12 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 				pend = 1;
 				dirty = true;
 			}
-			{ // record this step in the history ring
+			{ // record this step in the history ring (overwriting step T - HISTORY)
 				std::int64_t const w = ( T & ( HISTORY - 1 ) ) >> 6;
 				unsigned long long const bit = 1ull << ( T & 63 );
 				unsigned long long const word = v.hring[w * N64 + i];
-				v.hring[w * N64 + i] = spiked ? ( word | bit ) : ( word & ~bit );
+				unsigned long long const nw = spiked ? ( word | bit ) : ( word & ~bit );
+				if( nw != word ) v.hring[w * N64 + i] = nw;
 				if( spiked ) v.last_spk[i] = std::int32_t( T );
 			}
 			if constexpr( has_neuron_traces_v<S> )
@@ -806,8 +926,9 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 				v.yh[slot * N64 + i] = y;
 			}
 			// Rows are flushed before their window overflows; a row never processed yet gets its
-			// first flush at step (i mod FLUSH_AGE), which spreads flushes evenly over steps.
-			if( v.hasp[i] && ( T - synced >= FLUSH_AGE || ( synced < 0 && T >= i % FLUSH_AGE ) ) )
+			// first flush at step (i mod age), which spreads flushes evenly over steps.
+			constexpr int age = flush_age_v<S>;
+			if( v.hasp[i] && ( T - synced >= age || ( synced < 0 && T >= i % age ) ) )
 			{
 				flush = true;
 				fl_synced = synced;
@@ -917,9 +1038,11 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 		std::int32_t const pend = uniform( v.fl_pend[f] );
 		std::int64_t const r0 = v.row_ptr[s];
 		std::int64_t const lo = r0 + v.pl_lo[s], hi = r0 + v.pl_hi[s];
-		std::int64_t const chunk = ( ( hi - lo + v.fparts - 1 ) / v.fparts + 255 ) & ~std::int64_t( 255 );
+		std::int64_t const chunk = ( ( hi - lo + v.fparts - 1 ) / v.fparts + 63 ) & ~std::int64_t( 63 );
 		std::int64_t const beg = lo + p * chunk;
 		std::int64_t const end = beg + chunk < hi ? beg + chunk : hi;
+		// the source's pre trace at its pending event (one value per row)
+		float const xf = has_neuron_traces_v<S> && pend ? v.xh[std::int64_t( ( synced + 1 ) & ( HISTORY - 1 ) ) * v.N + s] : 0.0f;
 		for( std::int64_t base = beg; base < end; base += 64 * U )
 		{
 			std::int32_t j[U];
@@ -936,22 +1059,47 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 			std::int32_t ls[U];
 #pragma unroll
 			for( int u = 0; u < U; u++ ) ls[u] = ok[u] ? v.last_spk[j[u]] : -1;
+			if constexpr( has_neuron_traces_v<S> )
+			{
+				// events only: load the state of every synapse with an event first, then replay
+				bool ev[U];
+				typename S::tuple_t sy[U];
+				traced_pre tp[U];
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					ev[u] = ok[u] && ( pend || ls[u] > synced );
+					if( ev[u] )
+					{
+						load_tuple<S>( sy[u], v.sa, base + lane + 64 * u );
+						tp[u] = load_traced( v, j[u], ls[u], synced, pend );
+					}
+				}
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					if( !ev[u] ) continue;
+					std::int64_t const e = base + lane + 64 * u;
+					auto bak = make_backend<FIXED>(
+					    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
+					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e ) };
+					replay_traced_pre<S>( itr, v, s, j[u], std::int32_t( T ), synced, pend, xf, tp[u], info, bak );
+					store_tuple<S>( sy[u], v.sa, e );
+					touched++;
+				}
+				continue;
+			}
 #pragma unroll
 			for( int u = 0; u < U; u++ )
 			{
 				if( !ok[u] ) continue;
-				if constexpr( has_neuron_traces_v<S> )
-					if( !pend && ls[u] <= synced ) continue; // no event for this synapse in the window
 				std::int64_t const e = base + lane + 64 * u;
 				typename S::tuple_t sy;
 				load_tuple<S>( sy, v.sa, e );
 				auto bak = make_backend<FIXED>(
 				    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
 				reg_iter<typename S::tuple_t> itr{ sy, std::uint32_t( e ) };
-				if constexpr( has_neuron_traces_v<S> )
-					replay_traced<S>( itr, v, s, j[u], std::int32_t( T ), synced, pend, info, bak );
-				else
-					replay<S>( itr, v, j[u], std::int32_t( T ), synced, pend, dt, info, bak );
+				replay<S>( itr, v, j[u], std::int32_t( T ), synced, pend, dt, info, bak );
 				store_tuple<S>( sy, v.sa, e );
 				touched++;
 			}

@@ -34,6 +34,11 @@ struct net_params
 	float param[4];
 };
 
+struct launch_cfg
+{
+	std::int32_t tile_w = 0, flush_slots = 0, flush_grid = 0;
+};
+
 // Model-independent interface used by the C ABI.
 struct net_iface
 {
@@ -69,6 +74,13 @@ struct net_iface
 	virtual std::uint64_t read_spike_log( std::uint64_t * off, std::int32_t * ids, std::uint64_t cap ) = 0;
 	virtual std::uint64_t device_bytes() const = 0;
 	virtual std::int64_t host_step() const = 0;
+
+	// Launch shapes (see hip_net::apply_launch) and a copy of the net (graph, neuron and
+	// synapse state as stored; clock and plasticity bookkeeping start afresh) for tuning them.
+	virtual launch_cfg get_launch() const = 0;
+	virtual void set_launch( launch_cfg c ) = 0;
+	virtual net_iface * clone() const = 0;
+	virtual bool is_plastic() const = 0;
 };
 
 inline int device_cu_count()
@@ -421,15 +433,58 @@ public:
 		return b;
 	}
 
+	// ---- launch shapes ----
+	launch_cfg get_launch() const override { return _lc; }
+	void set_launch( launch_cfg c ) override
+	{
+		synchronize();
+		apply_launch( c );
+	}
+	bool is_plastic() const override { return plastic; }
+
+	net_iface * clone() const override
+	{
+		// the original is only read (its lazily updated synapses are copied as they are)
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+		hipStream_t s = nullptr;
+		SPICE_HIP( hipStreamCreateWithFlags( &s, hipStreamNonBlocking ) );
+		try
+		{
+			csr_dev g;
+			g.n = _g.n;
+			g.nnz = _g.nnz;
+			g.width = _g.width;
+			g.row_ptr.resize( _g.row_ptr.size() );
+			g.col.resize( _g.col.size() );
+			SPICE_HIP( hipMemcpyAsync( g.row_ptr.data(), _g.row_ptr.data(), _g.row_ptr.bytes(), hipMemcpyDeviceToDevice, s ) );
+			SPICE_HIP( hipMemcpyAsync( g.col.data(), _g.col.data(), _g.col.bytes(), hipMemcpyDeviceToDevice, s ) );
+			SPICE_HIP( hipStreamSynchronize( s ) );
+			auto * c = new hip_net( _p, std::move( g ), s );
+			s = nullptr; // owned by c
+			for( int a = 0; a < MAXA; a++ )
+			{
+				if( _na[a].bytes() ) SPICE_HIP( hipMemcpy( c->_na[a].data(), _na[a].data(), _na[a].bytes(), hipMemcpyDeviceToDevice ) );
+				if( _sa[a].bytes() ) SPICE_HIP( hipMemcpy( c->_sa[a].data(), _sa[a].data(), _sa[a].bytes(), hipMemcpyDeviceToDevice ) );
+			}
+			c->apply_launch( _lc );
+			return c;
+		}
+		catch( ... )
+		{
+			if( s ) (void)hipStreamDestroy( s );
+			throw;
+		}
+	}
+
 	// ---- used by hip_batch (batched stepping of many nets) ----
 	dview const & view( int parity ) const { return _vs[parity & 1]; }
 	int num_tiles() const { return _v.ntiles; }
 	int num_update_blocks() const { return _grid_update; }
 	// flush workgroups of this net inside a batch: about two per expected work item wave
-	// (rows reach their flush about every FLUSH_AGE steps), at most the net's own grid
+	// (rows reach their flush about every flush_age_v steps), at most the net's own grid
 	int num_batch_flush_blocks() const
 	{
-		std::int64_t const items = ( _g.n / FLUSH_AGE + 1 ) * _v.fparts;
+		std::int64_t const items = ( _g.n / flush_age_v<S> + 1 ) * _v.fparts;
 		return int( std::clamp<std::int64_t>( ( items + 1 ) / 2, 1, _grid_flush ) );
 	}
 	bool fixed() const { return ( _p.flags & SPICE_FLAG_FIXED_POINT ) != 0; }
@@ -465,7 +520,8 @@ private:
 	dview _v{};      // step views: _vs[p] reads clock slot p
 	dview _vs[2]{};
 
-	int _grid_receive = 0, _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
+	int _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
+	launch_cfg _lc{};
 	dbuf<std::int32_t> _pivots;
 	std::int64_t _step = 0;
 	util::kahan_sum<float> _clock;
@@ -579,35 +635,88 @@ private:
 		v.ctl = _ctl.data();
 		v.ctl_next = _ctl.data() + 1;
 
-		// Launch shapes. Rows longer than ~1k targets are split so one delivered spike
-		// occupies several waves; grids are persistent (work is read from device memory).
-		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
-		v.parts = std::max( 1, int( mean_row / 1024.0 + 0.5 ) );
-		v.fparts = std::max( 1, int( mean_row / 256.0 + 0.5 ) );
-		int const cus = device_cu_count();
 		_grid_update = int( ( n + 255 ) / 256 );
-		// persistent grids: up to 8 waves per SIMD (the kernels use few VGPRs)
-		_grid_receive = int( std::min<std::int64_t>( 8 * cus, std::max<std::int64_t>( 1, ( n * v.parts + 3 ) / 4 ) ) );
-		_grid_flush = int( std::min<std::int64_t>( 8 * cus, std::max<std::int64_t>( 1, ( n * v.fparts + 3 ) / 4 ) ) );
-		// target tiles for spike delivery: ~2 per CU, 256 .. TILE_MAX targets each
-		v.tile_w = int( std::min<std::int64_t>( TILE_MAX, std::max<std::int64_t>( 256, ( ( n + 2 * cus - 1 ) / ( 2 * cus ) + 63 ) / 64 * 64 ) ) );
-		v.ntiles = int( ( n + v.tile_w - 1 ) / v.tile_w );
-		_pivots.resize( std::size_t( n ) * ( v.ntiles + 1 ) );
-		v.pivots = _pivots.data();
-		_grid_tiles = std::min( v.ntiles, 16 * cus );
-		int const max_blocks = std::max( { _grid_update, _grid_receive, _grid_flush, v.ntiles } ); // receive slots are tiles
-		_bstats.resize( std::size_t( max_blocks ) * NSTAT );
-		SPICE_HIP( hipMemsetAsync( _bstats.data(), 0, _bstats.bytes(), _s ) );
-		v.bstats = _bstats.data();
+		apply_launch( launch_cfg{} );
 	}
 
-	void initialize()
+	// Launch shapes (zero fields: the defaults). tile_w: targets per delivery tile (the
+	// receive workgroup's LDS accumulators; ~2 tiles per CU by default); flush_slots: target
+	// slots per flush work item (one wave), so a flushed row of length L is split into about
+	// L / flush_slots items; flush_grid: persistent flush workgroups (8 per CU by default).
+	launch_cfg default_launch() const
 	{
+		int const cus = device_cu_count();
+		std::int64_t const n = _g.n;
+		launch_cfg c;
+		c.tile_w = int( std::min<std::int64_t>( TILE_MAX, std::max<std::int64_t>( 256, ( ( n + 2 * cus - 1 ) / ( 2 * cus ) + 63 ) / 64 * 64 ) ) );
+		c.flush_slots = 256;
+		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
+		std::int64_t const fparts = std::max( 1, int( mean_row / c.flush_slots + 0.5 ) );
+		c.flush_grid = int( std::min<std::int64_t>( 8 * cus, std::max<std::int64_t>( 1, ( n * fparts + 3 ) / 4 ) ) );
+		return c;
+	}
+
+	void apply_launch( launch_cfg c )
+	{
+		launch_cfg const d = default_launch();
+		if( c.tile_w == 0 ) c.tile_w = d.tile_w;
+		if( c.flush_slots == 0 ) c.flush_slots = d.flush_slots;
+		if( c.flush_grid == 0 ) c.flush_grid = d.flush_grid;
+		int const cus = device_cu_count();
+		if( c.tile_w < 64 || c.tile_w > TILE_MAX || c.tile_w % 64 )
+			throw std::invalid_argument( "tile_w must be a multiple of 64 in [64, 1024]" );
+		if( c.flush_slots < 64 || c.flush_slots > ( 1 << 20 ) || c.flush_slots % 64 )
+			throw std::invalid_argument( "flush_slots must be a multiple of 64 in [64, 2^20]" );
+		if( c.flush_grid < 1 || c.flush_grid > 64 * cus ) throw std::invalid_argument( "flush_grid must be in [1, 64 x CUs]" );
+		std::int64_t const n = _g.n;
+		dview & v = _v;
+		bool const retile = c.tile_w != v.tile_w;
+		if( retile )
+		{
+			std::int32_t const ntiles = std::int32_t( ( n + c.tile_w - 1 ) / c.tile_w );
+			_pivots.resize( std::size_t( n ) * ( ntiles + 1 ) );
+			v.tile_w = c.tile_w;
+			v.ntiles = ntiles;
+			v.pivots = _pivots.data();
+		}
+		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
+		v.fparts = std::max( 1, int( mean_row / c.flush_slots + 0.5 ) );
+		_grid_flush = c.flush_grid;
+		_grid_tiles = std::min( v.ntiles, 16 * cus );
+		_lc = c;
+		// statistics slots: one per workgroup of the widest step kernel (receive: per tile)
+		std::size_t const slots = std::size_t( std::max( { _grid_update, _grid_flush, v.ntiles } ) ) * NSTAT;
+		if( slots > _bstats.size() )
+		{
+			std::vector<unsigned long long> keep( NSTAT, 0ull );
+			if( _bstats.size() )
+			{
+				std::vector<unsigned long long> b( _bstats.size() );
+				SPICE_HIP( hipMemcpyAsync( b.data(), _bstats.data(), _bstats.bytes(), hipMemcpyDeviceToHost, _s ) );
+				SPICE_HIP( hipStreamSynchronize( _s ) );
+				for( std::size_t i = 0; i < b.size(); i++ ) keep[i % NSTAT] += b[i];
+			}
+			_bstats.resize( slots );
+			SPICE_HIP( hipMemsetAsync( _bstats.data(), 0, _bstats.bytes(), _s ) );
+			SPICE_HIP( hipMemcpyAsync( _bstats.data(), keep.data(), NSTAT * 8, hipMemcpyHostToDevice, _s ) );
+			SPICE_HIP( hipStreamSynchronize( _s ) ); // `keep` is a host temporary
+			v.bstats = _bstats.data();
+		}
 		_vs[0] = _v;
 		_vs[1] = _v;
 		_vs[1].ctl = _ctl.data() + 1;
 		_vs[1].ctl_next = _ctl.data();
-		SPICE_LAUNCH( k_pivots<<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
+		if( retile ) SPICE_LAUNCH( k_pivots<<<unsigned( ( n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
+		if( _exec )
+		{
+			(void)hipGraphExecDestroy( _exec ); // the captured launches hold the old shapes
+			_exec = nullptr;
+		}
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+	}
+
+	void initialize()
+	{
 		SPICE_LAUNCH( k_init_neurons<Model><<<_grid_update, 256, 0, _s>>>( _v ) );
 		if constexpr( plastic )
 		{

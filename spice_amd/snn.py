@@ -237,6 +237,27 @@ class Snn:
     def set_clock(self, step, sum_, comp):
         _lib.check(_lib.lib().spice_hip_set_clock(self._h, int(step), float(sum_), float(comp)))
 
+    # ---- launch shapes ----
+    def launch(self):
+        """Current launch shapes: tile_w (targets per delivery tile), flush_slots (target slots
+        per plasticity-flush work item), flush_grid (flush workgroups)."""
+        c = _lib.Launch()
+        _lib.check(_lib.lib().spice_hip_get_launch(self._h, _c.byref(c)))
+        return {"tile_w": c.tile_w, "flush_slots": c.flush_slots, "flush_grid": c.flush_grid}
+
+    def set_launch(self, tile_w=0, flush_slots=0, flush_grid=0):
+        """Sets launch shapes (0 = default); results do not depend on them."""
+        c = _lib.Launch(int(tile_w), int(flush_slots), int(flush_grid), 0)
+        _lib.check(_lib.lib().spice_hip_set_launch(self._h, _c.byref(c)))
+
+    def autotune(self, warmup=500, steps=100):
+        """Times candidate launch shapes on a copy of this network and applies the fastest;
+        returns (launch dict, step time in us). The network itself does not advance."""
+        c = _lib.Launch()
+        us = _c.c_double()
+        _lib.check(_lib.lib().spice_hip_autotune(self._h, int(warmup), int(steps), _c.byref(c), _c.byref(us)))
+        return {"tile_w": c.tile_w, "flush_slots": c.flush_slots, "flush_grid": c.flush_grid}, us.value
+
     def stats(self):
         s = _lib.Stats()
         _lib.check(_lib.lib().spice_hip_get_stats(self._h, _c.byref(s)))

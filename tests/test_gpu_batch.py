@@ -51,14 +51,14 @@ def test_batch_equals_networks_alone(gpu, model):
 def test_plastic_batch_equals_networks_alone(gpu, model):
     """Plastic nets in one batch (batched receive, update and lazy-STDP flush launches): spikes,
     neuron state, synaptic weights and counters equal each net run alone, over enough steps
-    for every row to be flushed at least once (window 255 steps)."""
+    for every row to be flushed several times (window 255 steps)."""
     rng = np.random.default_rng(11)
     nets = []
     for k in range(5):
         n = int(rng.integers(400, 2500))
         lay = three_pop_layout(n) if model == "brunel_3pop_stdp" else ([n // 2, n - n // 2], [(0, 1, 0.1), (1, 1, 0.1)])
         nets.append((Layout(*lay), int(rng.choice([1, 4, 15])), 500 + k, None))
-    steps = 600
+    steps = 700
     b = Batch(model, nets, 1e-4, fixed_point=True)
     b.run(steps)
     b.synchronize()

@@ -22,7 +22,7 @@ def _jobs():
     return [
         ("brunel_3pop", Layout(*three_pop_layout(1500)), 700, 15, 101, [12.0]),
         ("vogels_abbott", Layout([2000], [(0, 0, 0.05)]), 450, 1, 102, None),
-        ("brunel_3pop_stdp", Layout(*three_pop_layout(1200)), 640, 15, 103, None),
+        ("brunel_3pop_stdp", Layout(*three_pop_layout(1200)), 800, 15, 103, None),  # past 255-step flushes
         ("brunel_3pop", Layout(*three_pop_layout(800)), 333, 4, 104, [25.0]),
         ("vogels_abbott", Layout([1200], [(0, 0, 0.05)]), 901, 1, 105, None),
         ("brunel_3pop_stdp", Layout(*three_pop_layout(600)), 280, 5, 106, None),
@@ -133,4 +133,4 @@ def test_invalid_jobs_are_rejected_at_submit(gpu):
         with pytest.raises(ValueError):
             sv.submit("brunel_3pop", Layout(*three_pop_layout(500)), DT, 10, delay=0)
         with pytest.raises(ValueError):
-            sv.submit("brunel_3pop_stdp", Layout(*three_pop_layout(500)), DT, 10, delay=300)  # plastic delay limit
+            sv.submit("brunel_3pop_stdp", Layout(*three_pop_layout(500)), DT, 10, delay=256)  # plastic delay limit
