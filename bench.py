@@ -312,7 +312,7 @@ def main():
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        cfg, us = net.autotune(warmup=500, steps=100)
+        cfg, us = net.autotune(warmup=500, steps=300)
         tune = {"launch": cfg, "predicted_us_per_step": round(us, 3), "tune_s": round(time.perf_counter() - t0, 3)}
         print(f"[bench] autotuned {cfg} ({us:.1f} us/step) in {tune['tune_s']}s", file=sys.stderr, flush=True)
 

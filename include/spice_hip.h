@@ -256,9 +256,11 @@ typedef struct spice_launch
 int spice_hip_get_launch( spice_net * net, spice_launch * out );
 int spice_hip_set_launch( spice_net * net, const spice_launch * in );
 /* Times candidate launch shapes on a copy of the network (warmup steps first, then `steps`
- * steps per candidate, per-kernel HIP-event times), applies the fastest to the network and
- * returns it with its step time (us). The network itself does not advance. Results are
- * cached per (model, neurons, edges, delay, flags) for the life of the process. */
+ * graph-replayed steps per candidate, wall time per step: the flush of a step overlaps the
+ * next step, so shapes are judged together; tile width first, then the flush shape),
+ * applies the fastest to the network and returns it with its step time (us). The network
+ * itself does not advance. Results are cached per (model, neurons, edges, delay, flags) for
+ * the life of the process. */
 int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_launch * out, double * out_us_per_step );
 
 /* ---- serving: a request loop over device-resident simulation jobs ------------------------ */

@@ -39,13 +39,14 @@ public:
 	    , _owned( std::move( owned ) )
 	    , _s( stream )
 	    , _own_stream( own_stream )
+	    , _lane( plastic )
 	{
 		if( _nets.empty() ) throw std::invalid_argument( "empty batch" );
 		_fixed = _nets[0]->fixed();
 		for( auto * n : _nets )
 			if( n->fixed() != _fixed ) throw std::invalid_argument( "all nets of a batch use the same accumulation mode" );
 		std::vector<std::int32_t> tn, tl, bn, bl, fn, fl, fc;
-		std::vector<dview> v0, v1;
+		std::vector<dview> vs[CLOCK_SLOTS];
 		for( std::size_t k = 0; k < _nets.size(); k++ )
 		{
 			hip_net<Model> & n = *_nets[k];
@@ -69,9 +70,8 @@ public:
 					fc.push_back( nf );
 				}
 			}
-			// batch step t runs net step host_step + t: clock slot (host_step + t) & 1
-			v0.push_back( n.view( int( n.host_step() & 1 ) ) );
-			v1.push_back( n.view( int( ( n.host_step() + 1 ) & 1 ) ) );
+			// batch step t runs net step host_step + t: clock slot (host_step + t) mod 4
+			for( int p = 0; p < CLOCK_SLOTS; p++ ) vs[p].push_back( n.view( int( ( n.host_step() + p ) & ( CLOCK_SLOTS - 1 ) ) ) );
 		}
 		_ntiles = std::int32_t( tn.size() );
 		_nblocks = std::int32_t( bn.size() );
@@ -83,8 +83,7 @@ public:
 		upload( _flush_net, fn );
 		upload( _flush_local, fl );
 		upload( _flush_count, fc );
-		upload( _views[0], v0 );
-		upload( _views[1], v1 );
+		for( int p = 0; p < CLOCK_SLOTS; p++ ) upload( _views[p], vs[p] );
 		_grid_tiles = std::min( _ntiles, 16 * device_cu_count() );
 		for( auto * n : _nets ) n->synchronize();
 		SPICE_HIP( hipStreamSynchronize( _s ) );
@@ -93,6 +92,7 @@ public:
 	~hip_batch() override
 	{
 		if( _exec ) (void)hipGraphExecDestroy( _exec );
+		if( _lane.s ) (void)hipStreamSynchronize( _lane.s );
 		if( _s ) (void)hipStreamSynchronize( _s );
 		if( _s && _own_stream ) (void)hipStreamDestroy( _s );
 	}
@@ -105,10 +105,11 @@ public:
 			launch_step( _step++ );
 			done++;
 		};
-		if( ( _step & 1 ) && done < n ) eager();
+		while( ( _step & ( CLOCK_SLOTS - 1 ) ) && done < n ) eager();
 		if( n - done >= graph_steps )
 		{
 			if( !_exec ) capture();
+			_lane.join_into( _s );
 			for( ; done + graph_steps <= n; done += graph_steps )
 			{
 				SPICE_HIP( hipGraphLaunch( _exec, _s ) );
@@ -123,10 +124,14 @@ public:
 	{
 		for( auto * net : _nets ) net->note_spike_log( nsteps ); // validates nsteps first
 		std::int32_t const n = std::int32_t( _nets.size() );
-		SPICE_LAUNCH( k_batch_log_window<<<unsigned( ( n + 255 ) / 256 ), 256, 0, _s>>>( _views[_step & 1].data(), n, nsteps ) );
+		SPICE_LAUNCH( k_batch_log_window<<<unsigned( ( n + 255 ) / 256 ), 256, 0, _s>>>( _views[_step & ( CLOCK_SLOTS - 1 )].data(), n, nsteps ) );
 	}
 
-	void synchronize() override { SPICE_HIP( hipStreamSynchronize( _s ) ); }
+	void synchronize() override
+	{
+		if( _lane.s ) SPICE_HIP( hipStreamSynchronize( _lane.s ) );
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+	}
 	int size() const override { return int( _nets.size() ); }
 	net_iface * net( int i ) override
 	{
@@ -139,13 +144,14 @@ private:
 	std::vector<std::unique_ptr<hip_net<Model>>> _owned;
 	hipStream_t _s = nullptr;
 	bool _own_stream = false;
+	side_lane _lane; // plastic models: the flush of step T overlaps step T + 1
 	hipGraphExec_t _exec = nullptr;
 	bool _fixed = false;
 	std::int64_t _step = 0;
 	std::int32_t _ntiles = 0, _nblocks = 0, _nflush = 0;
 	int _grid_tiles = 0;
 	dbuf<std::int32_t> _tile_net, _tile_local, _block_net, _block_local, _flush_net, _flush_local, _flush_count;
-	dbuf<dview> _views[2];
+	dbuf<dview> _views[CLOCK_SLOTS];
 
 	template <typename T>
 	void upload( dbuf<T> & d, std::vector<T> const & h )
@@ -155,22 +161,30 @@ private:
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
-	void launch_step( std::int64_t t )
+	// Same schedule as hip_net::launch_step: the flush of step t overlaps step t + 1.
+	void launch_step( std::int64_t t, bool head = false )
 	{
 		if( _fixed )
-			launch_step_t<true>( t );
+			launch_step_t<true>( t, head );
 		else
-			launch_step_t<false>( t );
+			launch_step_t<false>( t, head );
 	}
 
 	template <bool F>
-	void launch_step_t( std::int64_t t )
+	void launch_step_t( std::int64_t t, bool head )
 	{
-		dview const * v = _views[t & 1].data();
+		dview const * v = _views[t & ( CLOCK_SLOTS - 1 )].data();
+		if constexpr( plastic )
+			if( !head ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
 		SPICE_LAUNCH( k_batch_receive<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v, _tile_net.data(), _tile_local.data(), _ntiles ) );
 		SPICE_LAUNCH( k_batch_update<Model, F><<<_nblocks, 256, 0, _s>>>( v, _block_net.data(), _block_local.data() ) );
 		if constexpr( plastic )
-			SPICE_LAUNCH( k_batch_flush<Model, F><<<_nflush, 256, 0, _s>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
+		{
+			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
+			SPICE_HIP( hipStreamWaitEvent( _lane.s, _lane.up, 0 ) );
+			SPICE_LAUNCH( k_batch_flush<Model, F><<<_nflush, 256, 0, _lane.s>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
+			SPICE_HIP( hipEventRecord( _lane.done[t & 1], _lane.s ) );
+		}
 	}
 
 	void capture()
@@ -179,7 +193,8 @@ private:
 		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
 		try
 		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step( i );
+			for( int i = 0; i < graph_steps; i++ ) launch_step( i, i < 2 );
+			_lane.join_into( _s );
 		}
 		catch( ... )
 		{

@@ -6,6 +6,7 @@
 #include <spice/util/layout.h>
 #include <spice_hip.h>
 
+#include <chrono>
 #include <climits>
 #include <limits>
 #include <map>
@@ -633,28 +634,31 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			std::unique_ptr<net_iface> c( n->clone() );
 			c->run( warmup );
 			c->synchronize();
-			// per-step time (ms) of kernel k (0 receive, 1 update, 2 flush) under cfg
-			auto time_of = [&]( launch_cfg cfg, int k ) {
+			// wall time per step (ms) of `steps` graph-replayed steps under cfg: the flush of a
+			// step overlaps the next step's delivery, so shapes are judged together
+			auto time_of = [&]( launch_cfg cfg ) {
 				c->set_launch( cfg );
-				double ms[3] = { 0, 0, 0 };
-				c->run_profiled( steps, ms );
-				return ms[k] / double( steps );
+				c->run( 32 ); // capture + warm the graph
+				c->synchronize();
+				auto const t0 = std::chrono::steady_clock::now();
+				c->run( steps );
+				c->synchronize();
+				return std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - t0 ).count() / double( steps );
 			};
 			launch_cfg cfg = c->get_launch();
-			double const inf = std::numeric_limits<double>::infinity();
-			double t_recv = inf, t_flush = 0.0;
+			double best_t = time_of( cfg );
 			std::int64_t const N = n->num_neurons();
 			for( int tw : { 128, 256, 384, 512, 768, 1024 } )
 			{
 				std::int64_t const tiles = ( N + tw - 1 ) / tw;
-				if( N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) ) continue; // pivots above 4 GB
+				if( tw == cfg.tile_w || N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) ) continue; // pivots above 4 GB
 				launch_cfg x = cfg;
 				x.tile_w = tw;
-				double const t = time_of( x, 0 );
-				if( t < t_recv )
+				double const t = time_of( x );
+				if( t < best_t )
 				{
-					t_recv = t;
-					cfg.tile_w = tw;
+					best_t = t;
+					cfg = x;
 				}
 			}
 			if( n->is_plastic() )
@@ -665,25 +669,23 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 					SPICE_HIP( hipGetDevice( &dev ) );
 					SPICE_HIP( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
 				}
-				t_flush = inf;
 				launch_cfg const base = cfg;
 				for( int fs : { 64, 128, 256, 512, 1024 } )
-					for( int fg : { 2, 4, 8, 16 } )
+					for( int fg : { 1, 2, 4, 8, 16 } )
 					{
 						launch_cfg x = base;
 						x.flush_slots = fs;
 						x.flush_grid = fg * cus;
-						double const t = time_of( x, 2 );
-						if( t < t_flush )
+						if( x.flush_slots == base.flush_slots && x.flush_grid == base.flush_grid ) continue;
+						double const t = time_of( x );
+						if( t < best_t )
 						{
-							t_flush = t;
-							cfg.flush_slots = fs;
-							cfg.flush_grid = x.flush_grid;
+							best_t = t;
+							cfg = x;
 						}
 					}
 			}
-			double const t_upd = time_of( cfg, 1 );
-			best = { cfg, ( t_recv + t_upd + t_flush ) * 1e3 };
+			best = { cfg, best_t * 1e3 };
 			std::lock_guard<std::mutex> g( g_tune_mutex );
 			g_tune_cache[key] = best;
 		}

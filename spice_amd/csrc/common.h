@@ -27,6 +27,51 @@ inline void check( hipError_t e, char const * what )
 		::spice::hipx::check( hipGetLastError(), "kernel launch" );   \
 	} while( 0 )
 
+// Second stream and events for work that overlaps the next step (the lazy-plasticity flush
+// of step T runs beside the delivery and update of step T + 1). Inactive if `on` is false.
+struct side_lane
+{
+	hipStream_t s = nullptr;
+	hipEvent_t up = nullptr, join = nullptr, done[2] = { nullptr, nullptr };
+
+	explicit side_lane( bool on )
+	{
+		if( !on ) return;
+		try
+		{
+			check( hipStreamCreateWithFlags( &s, hipStreamNonBlocking ), "hipStreamCreateWithFlags" );
+			for( hipEvent_t * e : { &up, &join, &done[0], &done[1] } )
+				check( hipEventCreateWithFlags( e, hipEventDisableTiming ), "hipEventCreateWithFlags" );
+		}
+		catch( ... )
+		{
+			release();
+			throw;
+		}
+	}
+	~side_lane() { release(); }
+	side_lane( side_lane const & ) = delete;
+	side_lane & operator=( side_lane const & ) = delete;
+
+	// main stream waits for everything enqueued on the lane so far
+	void join_into( hipStream_t main )
+	{
+		if( !s ) return;
+		check( hipEventRecord( join, s ), "hipEventRecord" );
+		check( hipStreamWaitEvent( main, join, 0 ), "hipStreamWaitEvent" );
+	}
+
+private:
+	void release()
+	{
+		if( s ) (void)hipStreamSynchronize( s );
+		for( hipEvent_t * e : { &up, &join, &done[0], &done[1] } )
+			if( *e ) (void)hipEventDestroy( *e );
+		if( s ) (void)hipStreamDestroy( s );
+		s = nullptr;
+	}
+};
+
 // Owning device allocation (reference counterpart: dbuffer, spice/cuda/util/dbuffer.h:17-90).
 template <typename T>
 class dbuf

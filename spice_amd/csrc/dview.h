@@ -17,8 +17,9 @@
 //   pend     int32[N]    row s has an undelivered pre event at synced+1
 //   hasp     uint8[N]    row s has at least one plastic synapse
 //   pl_lo/hi int32[N]    first / one-past-last plastic slot of row s (offsets in the row)
-//   flush_*  int32[N]    rows whose history window is full this step, with their
-//                        (synced, pend) at that time; counters double-buffered by step parity
+//   flush_*  int32[2][N] rows whose history window is full this step, with their
+//                        (synced, pend) at that time; lists by step parity, counters [4]
+//                        by step mod 4
 //   pivots   int32[N * (ntiles+1)]  per row and target tile: offset of the tile's first target
 //   xtr/ytr  f32[N]      neuron-local pre/post traces (trace synapses)
 //   xh/yh    f32[H * N]  their values over the last H steps (ring, slot = step mod H)
@@ -30,9 +31,11 @@ namespace spice::hipx
 {
 constexpr int MAXA = 16;
 
-// Device clock. Two slots: the kernels of step T read slot T & 1 and the neuron-update
-// kernel writes step T + 1's clock into the other slot, so the clock advances without
-// atomics or an extra kernel, and a captured graph of an even number of steps replays.
+// Device clock. Four slots: the kernels of step T read slot T & 3 and the neuron-update
+// kernel writes step T + 1's clock into slot (T + 1) & 3, so the clock advances without
+// atomics or an extra kernel, the flush of step T (which may run while step T + 1 delivers
+// and updates) still reads its own slot, and a captured graph of a multiple of 4 steps replays.
+constexpr int CLOCK_SLOTS = 4;
 struct ctl_t
 {
 	std::int64_t step; // index of the step being executed
@@ -93,7 +96,7 @@ struct dview
 	std::int32_t * fl_id;
 	std::int32_t * fl_synced;
 	std::int32_t * fl_pend;
-	std::uint32_t * fl_count; // [2]
+	std::uint32_t * fl_count; // [4]
 	float * xtr;
 	float * ytr;
 	float * xh;

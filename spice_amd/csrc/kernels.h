@@ -359,25 +359,31 @@ __device__ inline traced_pre load_traced( dview const & v, std::int32_t j, std::
 
 // replay_traced over preloaded data (same event sequence and arithmetic): only the traces at
 // the target's post spikes in the window are loaded here.
+// Returns whether any event was applied (a target whose latest spike is after `now` -- it
+// spiked in a step running concurrently -- may have none in the window).
 template <typename Syn, typename Iter, typename Bak>
-__device__ inline void replay_traced_pre(
+__device__ inline bool replay_traced_pre(
     Iter syn, dview const & v, std::int32_t s, std::int32_t j, std::int32_t now, std::int32_t synced, std::int32_t pend, float xf,
     traced_pre const & t, snn_info info, Bak & bak )
 {
 	std::int64_t const N = v.N;
 	std::int32_t first = synced + 1;
+	bool any = false;
 	if( pend )
 	{
 		bool const post = t.ls >= first && hbit( t.h, first );
 		Syn::on_event( syn, true, post, xf, t.yf, info, bak );
 		first++;
+		any = true;
 	}
 	std::int32_t const last = t.ls < now ? t.ls : now;
 	if( first <= last )
 		for_each_spike( t.h, first, last, [&]( std::int32_t e ) {
 			std::int64_t const slot = e & ( HISTORY - 1 );
 			Syn::on_event( syn, false, true, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
+			any = true;
 		} );
+	return any;
 }
 
 // Whether synapse src -> j has an event in (synced, now] (trace synapses only need those).
@@ -665,9 +671,9 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
 						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc };
 						reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
-						if( ev[u] )
+						if( ev[u] &&
+						    replay_traced_pre<S>( itr, v, s, j[u], std::int32_t( T - 1 ), seg_synced[kk[u]], seg_pend[kk[u]], seg_xf[kk[u]], tp[u], info, bak ) )
 						{
-							replay_traced_pre<S>( itr, v, s, j[u], std::int32_t( T - 1 ), seg_synced[kk[u]], seg_pend[kk[u]], seg_xf[kk[u]], tp[u], info, bak );
 							store_tuple<S>( sy[u], v.sa, e[u] );
 							pev++;
 						}
@@ -761,7 +767,8 @@ __global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
 	__shared__ std::int32_t wsum[TBS / 64];
 
 	std::int64_t const T = v.ctl->step;
-	std::int32_t const nrows = std::int32_t( v.fl_count[T & 1] );
+	std::int32_t const nrows = std::int32_t( v.fl_count[T & 3] );
+	std::int64_t const fb = ( T & 1 ) * std::int64_t( v.N ); // this step's flush list
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
 	int const tid = threadIdx.x;
@@ -775,7 +782,7 @@ __global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
 				std::int32_t len = 0;
 				if( tid < cnt )
 				{
-					std::int32_t const r = v.fl_id[c0 + tid];
+					std::int32_t const r = v.fl_id[fb + c0 + tid];
 					std::int32_t const * pv = v.pivots + std::int64_t( r ) * ( v.ntiles + 1 ) + tile;
 					std::int32_t a = pv[0], b = pv[1];
 					std::int32_t const pa = v.pl_lo[r], pb = v.pl_hi[r];
@@ -784,8 +791,8 @@ __global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
 					len = b > a ? b - a : 0;
 					seg_beg[tid] = v.row_ptr[r] + a;
 					seg_src[tid] = r;
-					seg_synced[tid] = v.fl_synced[c0 + tid];
-					seg_pend[tid] = v.fl_pend[c0 + tid];
+					seg_synced[tid] = v.fl_synced[fb + c0 + tid];
+					seg_pend[tid] = v.fl_pend[fb + c0 + tid];
 				}
 				std::int32_t const pre = block_exclusive_scan( len, wsum, &seg_pre[SPK_CHUNK] );
 				if( tid < cnt ) seg_pre[tid] = pre;
@@ -849,7 +856,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 	if( blk == 0 && threadIdx.x == 0 )
 	{
 		v.counts[( T + 1 ) % D1] = 0; // consumed by this step's receive; refilled at T + 1
-		if constexpr( plastic ) v.fl_count[( T + 1 ) & 1] = 0;
+		if constexpr( plastic ) v.fl_count[( T + 1 ) & 3] = 0; // step T - 1's flush may still read slot (T - 1) & 3
 		// next step's clock: exactly kahan_sum::add on the host
 		ctl_t const c = *v.ctl;
 		float const y = v.DT - c.c;
@@ -928,7 +935,10 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 			// Rows are flushed before their window overflows; a row never processed yet gets its
 			// first flush at step (i mod age), which spreads flushes evenly over steps.
 			constexpr int age = flush_age_v<S>;
-			if( v.hasp[i] && ( T - synced >= age || ( synced < 0 && T >= i % age ) ) )
+			// A row whose spike arrives at T + 1 is brought up to date by that delivery instead:
+			// the flush of step T may run while step T + 1 delivers, so their rows must differ.
+			bool const arrives_next = T + 1 >= v.D && ( v.D == 1 ? spiked : spiked_at( v, std::int32_t( i ), std::int32_t( T + 1 - v.D ) ) );
+			if( v.hasp[i] && !arrives_next && ( T - synced >= age || ( synced < 0 && T >= i % age ) ) )
 			{
 				flush = true;
 				fl_synced = synced;
@@ -970,7 +980,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 			t1 += b;
 		}
 		bbase[0] = t0 ? atomicAdd( &v.counts[wslot], t0 ) : 0u;
-		if constexpr( plastic ) bbase[1] = t1 ? atomicAdd( &v.fl_count[T & 1], t1 ) : 0u;
+		if constexpr( plastic ) bbase[1] = t1 ? atomicAdd( &v.fl_count[T & 3], t1 ) : 0u;
 		if( t0 ) v.bstats[blk * NSTAT + ST_SPIKES] += t0;
 	}
 	__syncthreads();
@@ -993,7 +1003,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 	{
 		if( flush )
 		{
-			unsigned const o = bbase[1] + wcount[w][1] + lanes_below( fm );
+			std::int64_t const o = ( T & 1 ) * std::int64_t( v.N ) + bbase[1] + wcount[w][1] + lanes_below( fm );
 			v.fl_id[o] = std::int32_t( i );
 			v.fl_synced[o] = fl_synced;
 			v.fl_pend[o] = fl_pend;
@@ -1022,7 +1032,8 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 	using S = typename Model::synapse;
 	constexpr int U = 4; // slots per lane per iteration: independent loads in flight
 	std::int64_t const T = v.ctl->step;
-	std::int64_t const nitems = std::int64_t( v.fl_count[T & 1] ) * v.fparts;
+	std::int64_t const nitems = std::int64_t( v.fl_count[T & 3] ) * v.fparts;
+	std::int64_t const fb = ( T & 1 ) * std::int64_t( v.N ); // this step's flush list
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
 	int const lane = lane_id();
@@ -1033,9 +1044,9 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 	{
 		std::int64_t const f = it / v.fparts;
 		std::int64_t const p = it - f * v.fparts;
-		std::int32_t const s = uniform( v.fl_id[f] );
-		std::int32_t const synced = uniform( v.fl_synced[f] );
-		std::int32_t const pend = uniform( v.fl_pend[f] );
+		std::int32_t const s = uniform( v.fl_id[fb + f] );
+		std::int32_t const synced = uniform( v.fl_synced[fb + f] );
+		std::int32_t const pend = uniform( v.fl_pend[fb + f] );
 		std::int64_t const r0 = v.row_ptr[s];
 		std::int64_t const lo = r0 + v.pl_lo[s], hi = r0 + v.pl_hi[s];
 		std::int64_t const chunk = ( ( hi - lo + v.fparts - 1 ) / v.fparts + 63 ) & ~std::int64_t( 63 );
@@ -1083,9 +1094,11 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 					auto bak = make_backend<FIXED>(
 					    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
 					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e ) };
-					replay_traced_pre<S>( itr, v, s, j[u], std::int32_t( T ), synced, pend, xf, tp[u], info, bak );
-					store_tuple<S>( sy[u], v.sa, e );
-					touched++;
+					if( replay_traced_pre<S>( itr, v, s, j[u], std::int32_t( T ), synced, pend, xf, tp[u], info, bak ) )
+					{
+						store_tuple<S>( sy[u], v.sa, e );
+						touched++;
+					}
 				}
 				continue;
 			}

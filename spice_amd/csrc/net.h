@@ -109,6 +109,7 @@ public:
 	    : _p( p )
 	    , _g( std::move( g ) )
 	    , _s( stream )
+	    , _lane( plastic )
 	{
 		if( !( p.dt > 0.0f ) ) throw std::invalid_argument( "dt must be positive" );
 		if( p.delay < 1 ) throw std::invalid_argument( "delay must be >= 1" );
@@ -123,6 +124,7 @@ public:
 	~hip_net() override
 	{
 		if( _exec ) (void)hipGraphExecDestroy( _exec );
+		if( _lane.s ) (void)hipStreamSynchronize( _lane.s );
 		if( _s ) (void)hipStreamSynchronize( _s );
 		if( _s ) (void)hipStreamDestroy( _s );
 	}
@@ -148,10 +150,11 @@ public:
 			_clock.add( _p.dt );
 			done++;
 		};
-		if( ( _step & 1 ) && done < n ) eager(); // graphs start at an even step
+		while( ( _step & ( CLOCK_SLOTS - 1 ) ) && done < n ) eager(); // graphs start at a step = 0 mod 4
 		if( n - done >= graph_steps )
 		{
 			if( !_exec ) capture();
+			_lane.join_into( _s ); // flushes of eager steps before the graph
 			for( ; done + graph_steps <= n; done += graph_steps )
 			{
 				SPICE_HIP( hipGraphLaunch( _exec, _s ) );
@@ -171,6 +174,7 @@ public:
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
 		constexpr int batch = 128, ev_per_step = 4;
+		synchronize(); // profiled steps run serially on the main stream
 		std::vector<hipEvent_t> ev( batch * ev_per_step );
 		for( auto & e : ev ) SPICE_HIP( hipEventCreate( &e ) );
 		try
@@ -182,9 +186,9 @@ public:
 				{
 					hipEvent_t * e = &ev[i * ev_per_step];
 					if( fixed() )
-						launch_step_events<true>( e, _vs[_step & 1] );
+						launch_step_events<true>( e, _vs[_step & ( CLOCK_SLOTS - 1 )] );
 					else
-						launch_step_events<false>( e, _vs[_step & 1] );
+						launch_step_events<false>( e, _vs[_step & ( CLOCK_SLOTS - 1 )] );
 					_step++;
 					_clock.add( _p.dt );
 				}
@@ -231,7 +235,11 @@ public:
 		}
 	}
 
-	void synchronize() override { SPICE_HIP( hipStreamSynchronize( _s ) ); }
+	void synchronize() override
+	{
+		if( _lane.s ) SPICE_HIP( hipStreamSynchronize( _lane.s ) );
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+	}
 
 	// ---- queries ----
 	net_params const & params() const override { return _p; }
@@ -253,7 +261,7 @@ public:
 		if( step < 0 || step > INT_MAX - 2 ) throw std::invalid_argument( "invalid step index" );
 		synchronize();
 		ctl_t ctl{ step, sum, c };
-		SPICE_HIP( hipMemcpy( _ctl.data() + ( step & 1 ), &ctl, sizeof( ctl ), hipMemcpyHostToDevice ) );
+		SPICE_HIP( hipMemcpy( _ctl.data() + ( step & ( CLOCK_SLOTS - 1 ) ), &ctl, sizeof( ctl ), hipMemcpyHostToDevice ) );
 		SPICE_HIP( hipMemset( _counts.data(), 0, _counts.bytes() ) );
 		if constexpr( plastic )
 		{
@@ -272,6 +280,7 @@ public:
 
 	void get_stats( spice_stats * out ) override
 	{
+		synchronize();
 		std::vector<unsigned long long> b( _bstats.size() );
 		SPICE_HIP( hipMemcpyAsync( b.data(), _bstats.data(), _bstats.bytes(), hipMemcpyDeviceToHost, _s ) );
 		SPICE_HIP( hipStreamSynchronize( _s ) );
@@ -282,6 +291,7 @@ public:
 	}
 	void reset_stats() override
 	{
+		synchronize();
 		SPICE_HIP( hipMemsetAsync( _bstats.data(), 0, _bstats.bytes(), _s ) );
 		synchronize();
 		_stats_step0 = _step;
@@ -379,7 +389,7 @@ public:
 		_slog_max = max_steps;
 		slog_t const w{ 0, 0, cap }; // empty window
 		SPICE_HIP( hipMemcpy( _slog_ctl.data(), &w, sizeof( w ), hipMemcpyHostToDevice ) );
-		for( dview * v : { &_v, &_vs[0], &_vs[1] } )
+		for( dview * v : { &_v, &_vs[0], &_vs[1], &_vs[2], &_vs[3] } )
 		{
 			v->slog = _slog_ctl.data();
 			v->slog_ids = _slog_ids.data();
@@ -445,7 +455,7 @@ public:
 	net_iface * clone() const override
 	{
 		// the original is only read (its lazily updated synapses are copied as they are)
-		SPICE_HIP( hipStreamSynchronize( _s ) );
+		const_cast<hip_net *>( this )->synchronize();
 		hipStream_t s = nullptr;
 		SPICE_HIP( hipStreamCreateWithFlags( &s, hipStreamNonBlocking ) );
 		try
@@ -477,7 +487,7 @@ public:
 	}
 
 	// ---- used by hip_batch (batched stepping of many nets) ----
-	dview const & view( int parity ) const { return _vs[parity & 1]; }
+	dview const & view( int slot ) const { return _vs[slot & ( CLOCK_SLOTS - 1 )]; }
 	int num_tiles() const { return _v.ntiles; }
 	int num_update_blocks() const { return _grid_update; }
 	// flush workgroups of this net inside a batch: about two per expected work item wave
@@ -502,6 +512,7 @@ private:
 	net_params _p;
 	csr_dev _g;
 	hipStream_t _s = nullptr;
+	side_lane _lane; // plastic models: the flush of step T overlaps step T + 1
 	hipGraphExec_t _exec = nullptr;
 
 	dbuf<char> _na[MAXA], _acc[MAXA], _sa[MAXA];
@@ -517,8 +528,8 @@ private:
 	dbuf<ctl_t> _ctl;
 	dbuf<unsigned long long> _bstats;
 	std::int64_t _stats_step0 = 0;
-	dview _v{};      // step views: _vs[p] reads clock slot p
-	dview _vs[2]{};
+	dview _v{};                // step views: _vs[p] reads clock slot p
+	dview _vs[CLOCK_SLOTS]{};
 
 	int _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
 	launch_cfg _lc{};
@@ -575,10 +586,10 @@ private:
 			_hasp.resize( n );
 			_pl_lo.resize( n );
 			_pl_hi.resize( n );
-			_fl_id.resize( n );
-			_fl_synced.resize( n );
-			_fl_pend.resize( n );
-			_fl_count.resize( 2 );
+			_fl_id.resize( 2 * n ); // lists of steps of both parities
+			_fl_synced.resize( 2 * n );
+			_fl_pend.resize( 2 * n );
+			_fl_count.resize( CLOCK_SLOTS );
 			SPICE_HIP( hipMemsetAsync( _hring.data(), 0, _hring.bytes(), _s ) );
 			SPICE_HIP( hipMemsetAsync( _last_spk.data(), 0xff, _last_spk.bytes(), _s ) ); // -1
 			SPICE_HIP( hipMemsetAsync( _pend.data(), 0, _pend.bytes(), _s ) );
@@ -595,7 +606,7 @@ private:
 			SPICE_HIP( hipMemcpyAsync( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice, _s ) );
 			SPICE_HIP( hipStreamSynchronize( _s ) ); // `synced` is a host temporary
 		}
-		_ctl.resize( 2 );
+		_ctl.resize( CLOCK_SLOTS );
 		SPICE_HIP( hipMemsetAsync( _ctl.data(), 0, _ctl.bytes(), _s ) );
 
 		dview & v = _v;
@@ -702,10 +713,12 @@ private:
 			SPICE_HIP( hipStreamSynchronize( _s ) ); // `keep` is a host temporary
 			v.bstats = _bstats.data();
 		}
-		_vs[0] = _v;
-		_vs[1] = _v;
-		_vs[1].ctl = _ctl.data() + 1;
-		_vs[1].ctl_next = _ctl.data();
+		for( int p = 0; p < CLOCK_SLOTS; p++ )
+		{
+			_vs[p] = _v;
+			_vs[p].ctl = _ctl.data() + p;
+			_vs[p].ctl_next = _ctl.data() + ( ( p + 1 ) & ( CLOCK_SLOTS - 1 ) );
+		}
 		if( retile ) SPICE_LAUNCH( k_pivots<<<unsigned( ( n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
 		if( _exec )
 		{
@@ -725,26 +738,35 @@ private:
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
-	// Enqueues the kernels of step `t` (its parity selects the clock slot).
-	void launch_step( std::int64_t t )
+	// Enqueues the kernels of step `t` (t mod 4 selects the clock slot). Plastic models: the
+	// flush of step t goes to the side lane after step t's update and overlaps step t + 1;
+	// step t's delivery first waits for the flush of step t - 2 (`head`: the first two steps
+	// of a captured graph, whose earlier flushes finished before the graph started).
+	void launch_step( std::int64_t t, bool head = false )
 	{
+		dview const & v = _vs[t & ( CLOCK_SLOTS - 1 )];
 		if( fixed() )
-			launch_step_t<true>( _vs[t & 1] );
+			launch_step_t<true>( v, t, head );
 		else
-			launch_step_t<false>( _vs[t & 1] );
+			launch_step_t<false>( v, t, head );
 	}
 
 	template <bool F>
-	void launch_step_t( dview const & v )
+	void launch_step_t( dview const & v, std::int64_t t, bool head )
 	{
+		if constexpr( plastic )
+			if( !head ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
 		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		if constexpr( plastic )
 		{
+			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
+			SPICE_HIP( hipStreamWaitEvent( _lane.s, _lane.up, 0 ) );
 			if( _p.flags & SPICE_FLAG_FLUSH_TILED )
-				SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
+				SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, _lane.s>>>( v ) );
 			else
-				SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( v ) );
+				SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _lane.s>>>( v ) );
+			SPICE_HIP( hipEventRecord( _lane.done[t & 1], _lane.s ) );
 		}
 	}
 
@@ -772,7 +794,8 @@ private:
 		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
 		try
 		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step( i ); // parity 0, 1, 0, ...
+			for( int i = 0; i < graph_steps; i++ ) launch_step( i, i < 2 ); // clock slots 0, 1, 2, 3, 0, ...
+			_lane.join_into( _s );                                          // the graph ends with its flushes
 		}
 		catch( ... )
 		{
@@ -789,6 +812,7 @@ private:
 	// Brings all plastic synapses up to date through the last completed step.
 	void sync_synapses()
 	{
+		synchronize();
 		if constexpr( plastic )
 		{
 			if( _step > 0 )
