@@ -79,6 +79,18 @@ def algorithmic_bytes(model, n, st, steps):
     return {"receive": recv, "update": upd, "flush": fl}
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy calibration), or (None, None)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)[workload]["kernels"][kernel]
+        return int(t["fetch_bytes"] + t["write_bytes"]), "profiles/pmc_traffic.json"
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def sweep_nets(count, seed):
     """Parameter sweep of the batched configuration: `count` independent nets, alternately
     Brunel (brunel_3pop, p = 0.1, delay 15, Poisson rate ~ U[5, 30] Hz) and Vogels-Abbott
@@ -285,6 +297,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-seq-stride", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--launch", default=None,
+                    help="tile_w,flush_slots,flush_grid: fixed launch shapes instead of autotuning")
     ap.add_argument("--serve-jobs", type=int, default=64, help="jobs of the served stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, jobs per second")
     ap.add_argument("--serve-budget", type=float, default=250e9, help="HBM admission budget, bytes")
@@ -310,7 +324,10 @@ def main():
     E = net.num_edges()
     print(f"[bench] {args.workload}: N={n} E={E} setup {setup_s:.3f}s", file=sys.stderr, flush=True)
     tune = None
-    if not args.no_autotune:
+    if args.launch:
+        tw, fs, fg = (int(x) for x in args.launch.split(","))
+        net.set_launch(tw, fs, fg)
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         cfg, us = net.autotune(warmup=500, steps=300)
         tune = {"launch": cfg, "predicted_us_per_step": round(us, 3), "tune_s": round(time.perf_counter() - t0, 3)}
@@ -337,9 +354,10 @@ def main():
     avg_ms = ms[kern] / args.profile_steps
     achieved = bytes_per[kern] / (avg_ms * 1e-3) / 1e9
     kernel_names = {"receive": "k_receive", "update": "k_update", "flush": "k_flush"}
+    traffic, traffic_src = measured_traffic(args.workload, kernel_names[kern])
     roofline = {
         "bound": "hbm", "kernel": kernel_names[kern], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
         "bytes_per_launch": round(bytes_per[kern]), "avg_launch_us": round(avg_ms * 1e3, 3),
         "kernel_us_per_step": {k: round(v / args.profile_steps * 1e3, 3) for k, v in ms.items()},
     }

@@ -441,6 +441,9 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 }
 
 // ---- target-tiled spike delivery -----------------------------------------------------------
+#ifndef SPICE_RECV_UNROLL
+#define SPICE_RECV_UNROLL 2
+#endif
 constexpr int TILE_MAX = 1024; // targets per tile (LDS accumulators)
 constexpr int TBS = 1024;      // threads per tile workgroup (16 waves)
 constexpr int SPK_CHUNK = TBS; // rows (delivered spikes / flushed rows) staged in LDS per pass
@@ -563,7 +566,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 	constexpr bool plastic = is_plastic_v<Model>;
 	constexpr std::uint32_t rmask = receive_mask_v<N>;
 	constexpr int nacc = __builtin_popcount( rmask ) > 0 ? __builtin_popcount( rmask ) : 1;
-	constexpr int U = 2;
+	constexpr int U = SPICE_RECV_UNROLL; // delivered events per thread in flight
 
 	__shared__ unsigned long long acc[nacc * TILE_MAX];
 	__shared__ std::int64_t seg_beg[SPK_CHUNK];
