@@ -243,14 +243,15 @@ int spice_batch_net( spice_batch * batch, int32_t i, spice_net ** out_net );
 /* ---- launch shapes and the autotuner --------------------------------------------------- */
 /* No counterpart in the reference (its launch shapes are compile-time constants). tile_w:
  * targets per spike-delivery tile (multiple of 64, 64..1024); flush_slots: target slots per
- * lazy-plasticity flush work item (multiple of 64); flush_grid: flush workgroups. A zero
- * field selects the default. Launch shapes never change results. */
+ * lazy-plasticity flush work item (multiple of 64); flush_grid: flush workgroups;
+ * recv_block: threads per delivery workgroup. A zero field selects the default. Launch shapes
+ * never change results. */
 typedef struct spice_launch
 {
 	int32_t tile_w;
 	int32_t flush_slots;
 	int32_t flush_grid;
-	int32_t reserved;
+	int32_t recv_block; /* delivery workgroup size: 256 or 1024 threads */
 } spice_launch;
 
 int spice_hip_get_launch( spice_net * net, spice_launch * out );

@@ -30,6 +30,9 @@ class hip_batch final : public batch_iface
 {
 	static constexpr int graph_steps = 32;
 	static constexpr bool plastic = is_plastic_v<Model>;
+	// delivery workgroup size: most tiles of a sweep belong to small nets and carry few events,
+	// so more, smaller workgroups shorten the launch's critical path
+	static constexpr int recv_block = SPICE_BATCH_RECV_BLOCK;
 
 public:
 	// `nets` must outlive the batch unless they are also handed over in `owned`. The batch
@@ -84,7 +87,7 @@ public:
 		upload( _flush_local, fl );
 		upload( _flush_count, fc );
 		for( int p = 0; p < CLOCK_SLOTS; p++ ) upload( _views[p], vs[p] );
-		_grid_tiles = std::min( _ntiles, 16 * device_cu_count() );
+		_grid_tiles = std::min( _ntiles, ( recv_block == 256 ? 64 : 16 ) * device_cu_count() );
 		for( auto * n : _nets ) n->synchronize();
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
@@ -176,7 +179,7 @@ private:
 		dview const * v = _views[t & ( CLOCK_SLOTS - 1 )].data();
 		if constexpr( plastic )
 			if( !head ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
-		SPICE_LAUNCH( k_batch_receive<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v, _tile_net.data(), _tile_local.data(), _ntiles ) );
+		SPICE_LAUNCH( k_batch_receive<Model, F, recv_block><<<_grid_tiles, recv_block, 0, _s>>>( v, _tile_net.data(), _tile_local.data(), _ntiles ) );
 		SPICE_LAUNCH( k_batch_update<Model, F><<<_nblocks, 256, 0, _s>>>( v, _block_net.data(), _block_local.data() ) );
 		if constexpr( plastic )
 		{

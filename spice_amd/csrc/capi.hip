@@ -592,7 +592,7 @@ int spice_hip_get_launch( spice_net * net, spice_launch * out )
 	return guard( [&] {
 		require( out != nullptr, "null output" );
 		launch_cfg const c = impl( net )->get_launch();
-		*out = { c.tile_w, c.flush_slots, c.flush_grid, 0 };
+		*out = { c.tile_w, c.flush_slots, c.flush_grid, c.recv_block };
 	} );
 }
 
@@ -600,7 +600,7 @@ int spice_hip_set_launch( spice_net * net, spice_launch const * in )
 {
 	return guard( [&] {
 		require( in != nullptr, "null input" );
-		stepping_impl( net )->set_launch( { in->tile_w, in->flush_slots, in->flush_grid } );
+		stepping_impl( net )->set_launch( { in->tile_w, in->flush_slots, in->flush_grid, in->recv_block } );
 	} );
 }
 
@@ -648,19 +648,23 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			launch_cfg cfg = c->get_launch();
 			double best_t = time_of( cfg );
 			std::int64_t const N = n->num_neurons();
-			for( int tw : { 128, 256, 384, 512, 768, 1024 } )
-			{
-				std::int64_t const tiles = ( N + tw - 1 ) / tw;
-				if( tw == cfg.tile_w || N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) ) continue; // pivots above 4 GB
-				launch_cfg x = cfg;
-				x.tile_w = tw;
-				double const t = time_of( x );
-				if( t < best_t )
+			launch_cfg const start = cfg;
+			for( int bs : { 256, 1024 } )
+				for( int tw : { 128, 256, 384, 512, 768, 1024 } )
 				{
-					best_t = t;
-					cfg = x;
+					std::int64_t const tiles = ( N + tw - 1 ) / tw;
+					if( ( tw == start.tile_w && bs == start.recv_block ) || N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) )
+						continue; // the start shape, or pivots above 4 GB
+					launch_cfg x = start;
+					x.tile_w = tw;
+					x.recv_block = bs;
+					double const t = time_of( x );
+					if( t < best_t )
+					{
+						best_t = t;
+						cfg = x;
+					}
 				}
-			}
 			if( n->is_plastic() )
 			{
 				int cus = 0;
@@ -690,7 +694,7 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			g_tune_cache[key] = best;
 		}
 		n->set_launch( best.first );
-		if( out ) *out = { best.first.tile_w, best.first.flush_slots, best.first.flush_grid, 0 };
+		if( out ) *out = { best.first.tile_w, best.first.flush_slots, best.first.flush_grid, best.first.recv_block };
 		if( out_us ) *out_us = best.second;
 	} );
 }

@@ -444,9 +444,14 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 #ifndef SPICE_RECV_UNROLL
 #define SPICE_RECV_UNROLL 2
 #endif
+#ifndef SPICE_BATCH_RECV_BLOCK
+#define SPICE_BATCH_RECV_BLOCK 256
+#endif
 constexpr int TILE_MAX = 1024; // targets per tile (LDS accumulators)
-constexpr int TBS = 1024;      // threads per tile workgroup (16 waves)
-constexpr int SPK_CHUNK = TBS; // rows (delivered spikes / flushed rows) staged in LDS per pass
+constexpr int TBS = 1024;      // threads per tile workgroup of the tiled flush (16 waves)
+constexpr int SPK_CHUNK = TBS; // rows (flushed rows) staged in LDS per pass
+// Delivery workgroups have 256 or 1024 threads (a launch shape): they stage that many spikes
+// per pass in LDS.
 
 // Per row r and tile t: offset (within the row) of the first target >= t * tile_w, for
 // t = 0 .. ntiles (the last entry is the row length). One wave per row, binary searches.
@@ -521,8 +526,8 @@ struct tile_backend : dev_backend<FIXED>
 	}
 };
 
-// Exclusive scan of one int32 per thread over a TBS-thread block: returns this thread's
-// exclusive prefix and writes the block total to *total (LDS). Contains barriers.
+// Exclusive scan of one int32 per thread over the block (a multiple of 64 threads): returns
+// this thread's exclusive prefix and writes the block total to *total (LDS). Contains barriers.
 __device__ inline std::int32_t block_exclusive_scan( std::int32_t x, std::int32_t * wsum, std::int32_t * total )
 {
 	std::uint64_t const incl = wave_inclusive_scan( std::uint64_t( std::uint32_t( x ) ) );
@@ -531,7 +536,7 @@ __device__ inline std::int32_t block_exclusive_scan( std::int32_t x, std::int32_
 	__syncthreads();
 	std::int32_t wb = 0;
 	for( int w = 0; w < ( tid >> 6 ); w++ ) wb += wsum[w];
-	if( tid == TBS - 1 ) *total = wb + std::int32_t( incl );
+	if( tid == int( blockDim.x ) - 1 ) *total = wb + std::int32_t( incl );
 	__syncthreads();
 	return wb + std::int32_t( incl ) - x;
 }
@@ -558,7 +563,7 @@ __device__ inline std::int32_t segment_of( std::int32_t const * pre, std::int32_
 // owning spike found by binary search in LDS). receive() accumulates into LDS with LDS
 // atomics; at the end the tile's sums go to the global accumulators with plain stores:
 // one workgroup owns each tile, so no global atomics are needed.
-template <typename Model, bool FIXED>
+template <typename Model, bool FIXED, int BS>
 __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 {
 	using N = typename Model::neuron;
@@ -569,13 +574,13 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 	constexpr int U = SPICE_RECV_UNROLL; // delivered events per thread in flight
 
 	__shared__ unsigned long long acc[nacc * TILE_MAX];
-	__shared__ std::int64_t seg_beg[SPK_CHUNK];
-	__shared__ std::int32_t seg_pre[SPK_CHUNK + 1];
-	__shared__ std::int32_t seg_src[SPK_CHUNK];
-	__shared__ std::int32_t seg_synced[plastic ? SPK_CHUNK : 1];
-	__shared__ std::int32_t seg_pend[plastic ? SPK_CHUNK : 1];
-	__shared__ float seg_xf[plastic && has_neuron_traces_v<S> ? SPK_CHUNK : 1];
-	__shared__ std::int32_t wsum[TBS / 64];
+	__shared__ std::int64_t seg_beg[BS];
+	__shared__ std::int32_t seg_pre[BS + 1];
+	__shared__ std::int32_t seg_src[BS];
+	__shared__ std::int32_t seg_synced[plastic ? BS : 1];
+	__shared__ std::int32_t seg_pend[plastic ? BS : 1];
+	__shared__ float seg_xf[plastic && has_neuron_traces_v<S> ? BS : 1];
+	__shared__ std::int32_t wsum[BS / 64];
 
 	std::int64_t const T = v.ctl->step;
 	int const D1 = v.D + 1;
@@ -596,12 +601,12 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 	{
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
-		for( int x = tid; x < nacc * TILE_MAX; x += TBS ) acc[x] = 0ull;
+		for( int x = tid; x < nacc * TILE_MAX; x += BS ) acc[x] = 0ull;
 		__syncthreads();
 
-		for( std::int32_t c0 = 0; c0 < nspk; c0 += SPK_CHUNK )
+		for( std::int32_t c0 = 0; c0 < nspk; c0 += BS )
 		{
-			std::int32_t const cnt = ( nspk - c0 ) < SPK_CHUNK ? ( nspk - c0 ) : SPK_CHUNK;
+			std::int32_t const cnt = ( nspk - c0 ) < BS ? ( nspk - c0 ) : BS;
 			std::int32_t len = 0;
 			if( tid < cnt )
 			{
@@ -621,12 +626,12 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 				}
 			}
 			// exclusive block scan of the segment lengths
-			std::int32_t const pre = block_exclusive_scan( len, wsum, &seg_pre[SPK_CHUNK] );
+			std::int32_t const pre = block_exclusive_scan( len, wsum, &seg_pre[BS] );
 			if( tid < cnt ) seg_pre[tid] = pre;
 			__syncthreads();
-			std::int32_t const total = seg_pre[SPK_CHUNK];
+			std::int32_t const total = seg_pre[BS];
 
-			for( std::int32_t f0 = 0; f0 < total; f0 += TBS * U )
+			for( std::int32_t f0 = 0; f0 < total; f0 += BS * U )
 			{
 				std::int32_t kk[U];
 				std::int64_t e[U];
@@ -635,7 +640,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 #pragma unroll
 				for( int u = 0; u < U; u++ )
 				{
-					std::int32_t const f = f0 + tid + TBS * u;
+					std::int32_t const f = f0 + tid + BS * u;
 					ok[u] = f < total;
 					std::int32_t const a = segment_of( seg_pre, cnt, f );
 					kk[u] = a;
@@ -723,7 +728,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 				using A = std::conditional_t<std::is_floating_point_v<T>, std::conditional_t<FIXED, long long, float>, T>;
 				constexpr int a = __builtin_popcount( rmask & ( ( 1u << I ) - 1u ) );
 				A const * src = reinterpret_cast<A const *>( acc + a * TILE_MAX );
-				for( int x = tid; x < tw; x += TBS )
+				for( int x = tid; x < tw; x += BS )
 				{
 					A const val = src[x];
 					if( val != A( 0 ) ) st<A>( v.acc[I], t0 + x, val );
@@ -735,20 +740,20 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 	block_add_stats<2>( v, tile, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
 }
 
-template <typename Model, bool FIXED>
-__global__ __launch_bounds__( TBS ) void k_receive_tiled( dview v )
+template <typename Model, bool FIXED, int BS>
+__global__ __launch_bounds__( BS ) void k_receive_tiled( dview v )
 {
-	for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x ) receive_tile<Model, FIXED>( v, tile );
+	for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x ) receive_tile<Model, FIXED, BS>( v, tile );
 }
 
 // Batched delivery: global tile gt belongs to net tile_net[gt] (local tile tile_local[gt]);
 // the net's view is read from device memory.
-template <typename Model, bool FIXED>
-__global__ __launch_bounds__( TBS ) void k_batch_receive(
+template <typename Model, bool FIXED, int BS>
+__global__ __launch_bounds__( BS ) void k_batch_receive(
     dview const * views, std::int32_t const * tile_net, std::int32_t const * tile_local, std::int32_t ntiles )
 {
 	for( std::int32_t gt = blockIdx.x; gt < ntiles; gt += gridDim.x )
-		receive_tile<Model, FIXED>( views[tile_net[gt]], tile_local[gt] );
+		receive_tile<Model, FIXED, BS>( views[tile_net[gt]], tile_local[gt] );
 }
 
 

@@ -36,7 +36,7 @@ struct net_params
 
 struct launch_cfg
 {
-	std::int32_t tile_w = 0, flush_slots = 0, flush_grid = 0;
+	std::int32_t tile_w = 0, flush_slots = 0, flush_grid = 0, recv_block = 0;
 };
 
 // Model-independent interface used by the C ABI.
@@ -664,6 +664,8 @@ private:
 		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
 		std::int64_t const fparts = std::max( 1, int( mean_row / c.flush_slots + 0.5 ) );
 		c.flush_grid = int( std::min<std::int64_t>( 8 * cus, std::max<std::int64_t>( 1, ( n * fparts + 3 ) / 4 ) ) );
+		// small nets: few events per tile, so more and smaller delivery workgroups
+		c.recv_block = n <= 65536 ? 256 : 1024;
 		return c;
 	}
 
@@ -673,6 +675,8 @@ private:
 		if( c.tile_w == 0 ) c.tile_w = d.tile_w;
 		if( c.flush_slots == 0 ) c.flush_slots = d.flush_slots;
 		if( c.flush_grid == 0 ) c.flush_grid = d.flush_grid;
+		if( c.recv_block == 0 ) c.recv_block = d.recv_block;
+		if( c.recv_block != 256 && c.recv_block != 1024 ) throw std::invalid_argument( "recv_block must be 256 or 1024" );
 		int const cus = device_cu_count();
 		if( c.tile_w < 64 || c.tile_w > TILE_MAX || c.tile_w % 64 )
 			throw std::invalid_argument( "tile_w must be a multiple of 64 in [64, 1024]" );
@@ -693,7 +697,7 @@ private:
 		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
 		v.fparts = std::max( 1, int( mean_row / c.flush_slots + 0.5 ) );
 		_grid_flush = c.flush_grid;
-		_grid_tiles = std::min( v.ntiles, 16 * cus );
+		_grid_tiles = std::min( v.ntiles, ( c.recv_block == 256 ? 64 : 16 ) * cus );
 		_lc = c;
 		// statistics slots: one per workgroup of the widest step kernel (receive: per tile)
 		std::size_t const slots = std::size_t( std::max( { _grid_update, _grid_flush, v.ntiles } ) ) * NSTAT;
@@ -756,7 +760,7 @@ private:
 	{
 		if constexpr( plastic )
 			if( !head ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
-		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
+		launch_receive<F>( v );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		if constexpr( plastic )
 		{
@@ -771,10 +775,19 @@ private:
 	}
 
 	template <bool F>
+	void launch_receive( dview const & v )
+	{
+		if( _lc.recv_block == 256 )
+			SPICE_LAUNCH( k_receive_tiled<Model, F, 256><<<_grid_tiles, 256, 0, _s>>>( v ) );
+		else
+			SPICE_LAUNCH( k_receive_tiled<Model, F, 1024><<<_grid_tiles, 1024, 0, _s>>>( v ) );
+	}
+
+	template <bool F>
 	void launch_step_events( hipEvent_t * e, dview const & v )
 	{
 		SPICE_HIP( hipEventRecord( e[0], _s ) );
-		SPICE_LAUNCH( k_receive_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
+		launch_receive<F>( v );
 		SPICE_HIP( hipEventRecord( e[1], _s ) );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[2], _s ) );

@@ -238,16 +238,21 @@ class Snn:
         _lib.check(_lib.lib().spice_hip_set_clock(self._h, int(step), float(sum_), float(comp)))
 
     # ---- launch shapes ----
+    @staticmethod
+    def _launch_dict(c):
+        return {"tile_w": c.tile_w, "flush_slots": c.flush_slots, "flush_grid": c.flush_grid, "recv_block": c.recv_block}
+
     def launch(self):
         """Current launch shapes: tile_w (targets per delivery tile), flush_slots (target slots
-        per plasticity-flush work item), flush_grid (flush workgroups)."""
+        per plasticity-flush work item), flush_grid (flush workgroups), recv_block (threads per
+        delivery workgroup)."""
         c = _lib.Launch()
         _lib.check(_lib.lib().spice_hip_get_launch(self._h, _c.byref(c)))
-        return {"tile_w": c.tile_w, "flush_slots": c.flush_slots, "flush_grid": c.flush_grid}
+        return self._launch_dict(c)
 
-    def set_launch(self, tile_w=0, flush_slots=0, flush_grid=0):
+    def set_launch(self, tile_w=0, flush_slots=0, flush_grid=0, recv_block=0):
         """Sets launch shapes (0 = default); results do not depend on them."""
-        c = _lib.Launch(int(tile_w), int(flush_slots), int(flush_grid), 0)
+        c = _lib.Launch(int(tile_w), int(flush_slots), int(flush_grid), int(recv_block))
         _lib.check(_lib.lib().spice_hip_set_launch(self._h, _c.byref(c)))
 
     def autotune(self, warmup=500, steps=100):
@@ -256,7 +261,7 @@ class Snn:
         c = _lib.Launch()
         us = _c.c_double()
         _lib.check(_lib.lib().spice_hip_autotune(self._h, int(warmup), int(steps), _c.byref(c), _c.byref(us)))
-        return {"tile_w": c.tile_w, "flush_slots": c.flush_slots, "flush_grid": c.flush_grid}, us.value
+        return self._launch_dict(c), us.value
 
     def stats(self):
         s = _lib.Stats()
