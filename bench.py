@@ -352,7 +352,7 @@ def main():
     kern = max(ms, key=lambda k: ms[k])
     avg_ms = ms[kern] / args.profile_steps
     achieved = bytes_per[kern] / (avg_ms * 1e-3) / 1e9
-    kernel_names = {"receive": "k_receive", "update": "k_update", "flush": "k_flush"}
+    kernel_names = {"receive": "k_receive_tiled", "update": "k_update", "flush": "k_flush"}
     traffic, traffic_src = measured_traffic(args.workload, kernel_names[kern])
     roofline = {
         "bound": "hbm", "kernel": kernel_names[kern], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
