@@ -298,7 +298,7 @@ def main():
     ap.add_argument("--batch-seq-stride", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--launch", default=None,
-                    help="tile_w,flush_slots,flush_grid,recv_block: fixed launch shapes instead of autotuning")
+                    help="tile_w,flush_slots,flush_grid,recv_block,recv_parts: fixed launch shapes instead of autotuning")
     ap.add_argument("--serve-jobs", type=int, default=64, help="jobs of the served stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, jobs per second")
     ap.add_argument("--serve-budget", type=float, default=250e9, help="HBM admission budget, bytes")

@@ -83,7 +83,9 @@ struct synapse : util::type_list<Ts...>
 // The synapse state holds only what the rule needs per synapse (typically the weight).
 // on_event must leave the synapse unchanged when pre and post are both false; that is what
 // lets the GPU apply it only at events (lazily) with results identical to the dense
-// per-step definition. The synapse::update of the generic contract is not used.
+// per-step definition. It reads x only when post is set and y only when pre is set (the
+// lazy replay passes 0 for a trace the event does not read, so it is never loaded). The
+// synapse::update of the generic contract is not used.
 template <typename... Ts>
 struct trace_synapse : synapse<Ts...>
 {

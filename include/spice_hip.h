@@ -251,7 +251,9 @@ typedef struct spice_launch
 	int32_t tile_w;
 	int32_t flush_slots;
 	int32_t flush_grid;
-	int32_t recv_block; /* delivery workgroup size: 256 or 1024 threads */
+	int32_t recv_block; /* delivery workgroup size: 256, 512 or 1024 threads */
+	int32_t recv_parts; /* delivery workgroups per tile, 1..16 (the step's spikes split between them) */
+	int32_t reserved[3];
 } spice_launch;
 
 int spice_hip_get_launch( spice_net * net, spice_launch * out );

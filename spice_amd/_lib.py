@@ -49,7 +49,7 @@ class Stats(ctypes.Structure):
 
 class Launch(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("flush_slots", ctypes.c_int32), ("flush_grid", ctypes.c_int32),
-                ("recv_block", ctypes.c_int32)]
+                ("recv_block", ctypes.c_int32), ("recv_parts", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class JobDesc(ctypes.Structure):

@@ -53,7 +53,7 @@ public:
 		for( std::size_t k = 0; k < _nets.size(); k++ )
 		{
 			hip_net<Model> & n = *_nets[k];
-			for( int t = 0; t < n.num_tiles(); t++ )
+			for( int t = 0; t < n.num_tiles() * n.num_receive_parts(); t++ ) // (tile, part) work items
 			{
 				tn.push_back( std::int32_t( k ) );
 				tl.push_back( t );

@@ -592,7 +592,7 @@ int spice_hip_get_launch( spice_net * net, spice_launch * out )
 	return guard( [&] {
 		require( out != nullptr, "null output" );
 		launch_cfg const c = impl( net )->get_launch();
-		*out = { c.tile_w, c.flush_slots, c.flush_grid, c.recv_block };
+		*out = { c.tile_w, c.flush_slots, c.flush_grid, c.recv_block, c.recv_parts, { 0, 0, 0 } };
 	} );
 }
 
@@ -600,7 +600,7 @@ int spice_hip_set_launch( spice_net * net, spice_launch const * in )
 {
 	return guard( [&] {
 		require( in != nullptr, "null input" );
-		stepping_impl( net )->set_launch( { in->tile_w, in->flush_slots, in->flush_grid, in->recv_block } );
+		stepping_impl( net )->set_launch( { in->tile_w, in->flush_slots, in->flush_grid, in->recv_block, in->recv_parts } );
 	} );
 }
 
@@ -649,8 +649,17 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			double best_t = time_of( cfg );
 			std::int64_t const N = n->num_neurons();
 			launch_cfg const start = cfg;
-			for( int bs : { 256, 1024 } )
-				for( int tw : { 128, 256, 384, 512, 768, 1024 } )
+			auto trial = [&]( launch_cfg const & x ) {
+				double const t = time_of( x );
+				if( t < best_t )
+				{
+					best_t = t;
+					cfg = x;
+				}
+			};
+			// delivery: tile width x workgroup size, then the split of each tile's spikes
+			for( int bs : { 256, 512, 1024 } )
+				for( int tw : { 128, 256, 512, 1024 } )
 				{
 					std::int64_t const tiles = ( N + tw - 1 ) / tw;
 					if( ( tw == start.tile_w && bs == start.recv_block ) || N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) )
@@ -658,13 +667,19 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 					launch_cfg x = start;
 					x.tile_w = tw;
 					x.recv_block = bs;
-					double const t = time_of( x );
-					if( t < best_t )
-					{
-						best_t = t;
-						cfg = x;
-					}
+					trial( x );
 				}
+			{
+				launch_cfg const base = cfg;
+				for( int parts : { 2, 3, 4, 6 } )
+					for( int bs : { 256, 512, 1024 } )
+					{
+						launch_cfg x = base;
+						x.recv_parts = parts;
+						x.recv_block = bs;
+						trial( x );
+					}
+			}
 			if( n->is_plastic() )
 			{
 				int cus = 0;
@@ -681,12 +696,7 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 						x.flush_slots = fs;
 						x.flush_grid = fg * cus;
 						if( x.flush_slots == base.flush_slots && x.flush_grid == base.flush_grid ) continue;
-						double const t = time_of( x );
-						if( t < best_t )
-						{
-							best_t = t;
-							cfg = x;
-						}
+						trial( x );
 					}
 			}
 			best = { cfg, best_t * 1e3 };
@@ -694,7 +704,7 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			g_tune_cache[key] = best;
 		}
 		n->set_launch( best.first );
-		if( out ) *out = { best.first.tile_w, best.first.flush_slots, best.first.flush_grid, best.first.recv_block };
+		if( out ) *out = { best.first.tile_w, best.first.flush_slots, best.first.flush_grid, best.first.recv_block, best.first.recv_parts, { 0, 0, 0 } };
 		if( out_us ) *out_us = best.second;
 	} );
 }

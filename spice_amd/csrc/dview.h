@@ -72,6 +72,7 @@ struct dview
 	std::int32_t N, D, width, shift;
 	float DT;
 	std::int32_t fparts; // work items per flushed row
+	std::int32_t rparts; // delivery workgroups per tile (the step's spike list split between them)
 	std::uint32_t flags;
 	std::uint64_t seed;
 	float param[4]; // per-net model parameters (snn_info::param)

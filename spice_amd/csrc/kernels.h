@@ -310,12 +310,12 @@ __device__ inline bool replay_traced(
 	{
 		bool const post = ls >= first && spiked_at( v, j, first );
 		std::int64_t const slot = first & ( HISTORY - 1 );
-		Syn::on_event( syn, true, post, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
+		Syn::on_event( syn, true, post, post ? v.xh[slot * N + s] : 0.0f, v.yh[slot * N + j], info, bak );
 		first++;
 	}
 	for_each_spike( v, j, first, ls < now ? ls : now, [&]( std::int32_t e ) {
 		std::int64_t const slot = e & ( HISTORY - 1 );
-		Syn::on_event( syn, false, true, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
+		Syn::on_event( syn, false, true, v.xh[slot * N + s], 0.0f, info, bak ); // post only: y is not read
 	} );
 	return true;
 }
@@ -372,7 +372,7 @@ __device__ inline bool replay_traced_pre(
 	if( pend )
 	{
 		bool const post = t.ls >= first && hbit( t.h, first );
-		Syn::on_event( syn, true, post, xf, t.yf, info, bak );
+		Syn::on_event( syn, true, post, post ? xf : 0.0f, t.yf, info, bak );
 		first++;
 		any = true;
 	}
@@ -380,7 +380,7 @@ __device__ inline bool replay_traced_pre(
 	if( first <= last )
 		for_each_spike( t.h, first, last, [&]( std::int32_t e ) {
 			std::int64_t const slot = e & ( HISTORY - 1 );
-			Syn::on_event( syn, false, true, v.xh[slot * N + s], v.yh[slot * N + j], info, bak );
+			Syn::on_event( syn, false, true, v.xh[slot * N + s], 0.0f, info, bak ); // post only: y is not read
 			any = true;
 		} );
 	return any;
@@ -564,8 +564,12 @@ __device__ inline std::int32_t segment_of( std::int32_t const * pre, std::int32_
 // atomics; at the end the tile's sums go to the global accumulators with plain stores:
 // one workgroup owns each tile, so no global atomics are needed.
 template <typename Model, bool FIXED, int BS>
-__device__ inline void receive_tile( dview const & v, std::int32_t tile )
+__device__ inline void receive_tile( dview const & v, std::int32_t work )
 {
+	// work item = (tile, part): the tile's spike list is split into v.rparts parts, each
+	// delivered by its own workgroup (more workgroups in flight for few, large tiles)
+	std::int32_t const tile = work / v.rparts;
+	std::int32_t const part = work - tile * v.rparts;
 	using N = typename Model::neuron;
 	using S = typename Model::synapse;
 	constexpr bool plastic = is_plastic_v<Model>;
@@ -585,15 +589,17 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 	std::int64_t const T = v.ctl->step;
 	int const D1 = v.D + 1;
 	int const slot = int( ( T + 1 ) % D1 ); // ring entry of step T - D
-	std::int32_t const nspk = T < v.D ? 0 : std::int32_t( v.counts[slot] );
-	if( v.slog && tile == 0 && threadIdx.x == 0 )
+	std::int32_t const nall = T < v.D ? 0 : std::int32_t( v.counts[slot] );
+	std::int32_t const s_beg = std::int32_t( std::int64_t( nall ) * part / v.rparts );
+	std::int32_t const nspk = std::int32_t( std::int64_t( nall ) * ( part + 1 ) / v.rparts ) - s_beg;
+	if( v.slog && work == 0 && threadIdx.x == 0 )
 	{
 		// spike log: step T's ids start where step T-1's (final by now) end
 		std::int64_t const k = T - v.slog->t0;
 		if( k >= 0 && k < v.slog->nsteps )
 			v.slog_off[k] = k == 0 ? 0ull : v.slog_off[k - 1] + v.counts[( T - 1 ) % D1];
 	}
-	std::int32_t const * spk = v.ring + std::int64_t( slot ) * v.N;
+	std::int32_t const * spk = v.ring + std::int64_t( slot ) * v.N + s_beg;
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
 	int const tid = threadIdx.x;
@@ -720,7 +726,10 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 			__syncthreads(); // segments are overwritten by the next chunk
 		}
 
-		// the tile's sums -> global accumulators (this workgroup owns them; they are zero here)
+		// the tile's sums -> global accumulators (zero here). One part: this workgroup owns
+		// them, plain stores; several parts: atomic adds (integer and thus exact in fixed-point
+		// mode; in fp32 mode the parts' order varies, as LDS atomic order already does)
+		bool const owned = v.rparts == 1;
 		util::for_n<N::size>( [&]( auto I ) {
 			if constexpr( ( ( rmask >> I ) & 1u ) != 0 )
 			{
@@ -731,23 +740,30 @@ __device__ inline void receive_tile( dview const & v, std::int32_t tile )
 				for( int x = tid; x < tw; x += BS )
 				{
 					A const val = src[x];
-					if( val != A( 0 ) ) st<A>( v.acc[I], t0 + x, val );
+					if( val == A( 0 ) ) continue;
+					if( owned )
+						st<A>( v.acc[I], t0 + x, val );
+					else if constexpr( std::is_same_v<A, long long> )
+						atomicAdd( reinterpret_cast<unsigned long long *>( v.acc[I] ) + t0 + x, static_cast<unsigned long long>( val ) );
+					else
+						atomicAdd( static_cast<A *>( v.acc[I] ) + t0 + x, val );
 				}
 			}
 		} );
 		__syncthreads();
 	}
-	block_add_stats<2>( v, tile, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
+	block_add_stats<2>( v, work, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
 }
 
 template <typename Model, bool FIXED, int BS>
 __global__ __launch_bounds__( BS ) void k_receive_tiled( dview v )
 {
-	for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x ) receive_tile<Model, FIXED, BS>( v, tile );
+	std::int32_t const nwork = v.ntiles * v.rparts;
+	for( std::int32_t w = blockIdx.x; w < nwork; w += gridDim.x ) receive_tile<Model, FIXED, BS>( v, w );
 }
 
-// Batched delivery: global tile gt belongs to net tile_net[gt] (local tile tile_local[gt]);
-// the net's view is read from device memory.
+// Batched delivery: global work item gt belongs to net tile_net[gt] (local work item
+// tile_local[gt] = tile * rparts + part); the net's view is read from device memory.
 template <typename Model, bool FIXED, int BS>
 __global__ __launch_bounds__( BS ) void k_batch_receive(
     dview const * views, std::int32_t const * tile_net, std::int32_t const * tile_local, std::int32_t ntiles )
@@ -943,10 +959,14 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 			// Rows are flushed before their window overflows; a row never processed yet gets its
 			// first flush at step (i mod age), which spreads flushes evenly over steps.
 			constexpr int age = flush_age_v<S>;
-			// A row whose spike arrives at T + 1 is brought up to date by that delivery instead:
-			// the flush of step T may run while step T + 1 delivers, so their rows must differ.
+			// Rows go to this step's flush list (and are current through T afterwards) when
+			//  * their spike arrived now: the pending pre event of step T is applied right away,
+			//    while every such row reads the same, cache-resident trace slot of step T, or
+			//  * their history window is full.
+			// A row whose spike arrives at T + 1 is left to that delivery instead: the flush of
+			// step T may run while step T + 1 delivers, so their rows must differ.
 			bool const arrives_next = T + 1 >= v.D && ( v.D == 1 ? spiked : spiked_at( v, std::int32_t( i ), std::int32_t( T + 1 - v.D ) ) );
-			if( v.hasp[i] && !arrives_next && ( T - synced >= age || ( synced < 0 && T >= i % age ) ) )
+			if( v.hasp[i] && !arrives_next && ( pend || T - synced >= age || ( synced < 0 && T >= i % age ) ) )
 			{
 				flush = true;
 				fl_synced = synced;

@@ -36,7 +36,7 @@ struct net_params
 
 struct launch_cfg
 {
-	std::int32_t tile_w = 0, flush_slots = 0, flush_grid = 0, recv_block = 0;
+	std::int32_t tile_w = 0, flush_slots = 0, flush_grid = 0, recv_block = 0, recv_parts = 0;
 };
 
 // Model-independent interface used by the C ABI.
@@ -489,12 +489,14 @@ public:
 	// ---- used by hip_batch (batched stepping of many nets) ----
 	dview const & view( int slot ) const { return _vs[slot & ( CLOCK_SLOTS - 1 )]; }
 	int num_tiles() const { return _v.ntiles; }
+	int num_receive_parts() const { return _v.rparts; }
 	int num_update_blocks() const { return _grid_update; }
 	// flush workgroups of this net inside a batch: about two per expected work item wave
-	// (rows reach their flush about every flush_age_v steps), at most the net's own grid
+	// (rows reach their flush about every flush_age_v steps, and once after each arrival of
+	// their spike), at most the net's own grid
 	int num_batch_flush_blocks() const
 	{
-		std::int64_t const items = ( _g.n / flush_age_v<S> + 1 ) * _v.fparts;
+		std::int64_t const items = ( _g.n / flush_age_v<S> + _g.n / 256 + 1 ) * _v.fparts;
 		return int( std::clamp<std::int64_t>( ( items + 1 ) / 2, 1, _grid_flush ) );
 	}
 	bool fixed() const { return ( _p.flags & SPICE_FLAG_FIXED_POINT ) != 0; }
@@ -666,6 +668,7 @@ private:
 		c.flush_grid = int( std::min<std::int64_t>( 8 * cus, std::max<std::int64_t>( 1, ( n * fparts + 3 ) / 4 ) ) );
 		// small nets: few events per tile, so more and smaller delivery workgroups
 		c.recv_block = n <= 65536 ? 256 : 1024;
+		c.recv_parts = 1;
 		return c;
 	}
 
@@ -676,7 +679,10 @@ private:
 		if( c.flush_slots == 0 ) c.flush_slots = d.flush_slots;
 		if( c.flush_grid == 0 ) c.flush_grid = d.flush_grid;
 		if( c.recv_block == 0 ) c.recv_block = d.recv_block;
-		if( c.recv_block != 256 && c.recv_block != 1024 ) throw std::invalid_argument( "recv_block must be 256 or 1024" );
+		if( c.recv_parts == 0 ) c.recv_parts = d.recv_parts;
+		if( c.recv_block != 256 && c.recv_block != 512 && c.recv_block != 1024 )
+			throw std::invalid_argument( "recv_block must be 256, 512 or 1024" );
+		if( c.recv_parts < 1 || c.recv_parts > 16 ) throw std::invalid_argument( "recv_parts must be in [1, 16]" );
 		int const cus = device_cu_count();
 		if( c.tile_w < 64 || c.tile_w > TILE_MAX || c.tile_w % 64 )
 			throw std::invalid_argument( "tile_w must be a multiple of 64 in [64, 1024]" );
@@ -696,11 +702,12 @@ private:
 		}
 		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
 		v.fparts = std::max( 1, int( mean_row / c.flush_slots + 0.5 ) );
+		v.rparts = c.recv_parts;
 		_grid_flush = c.flush_grid;
-		_grid_tiles = std::min( v.ntiles, ( c.recv_block == 256 ? 64 : 16 ) * cus );
+		_grid_tiles = std::min( v.ntiles * v.rparts, ( 16 * 1024 / c.recv_block ) * cus );
 		_lc = c;
-		// statistics slots: one per workgroup of the widest step kernel (receive: per tile)
-		std::size_t const slots = std::size_t( std::max( { _grid_update, _grid_flush, v.ntiles } ) ) * NSTAT;
+		// statistics slots: one per workgroup of the widest step kernel (receive: per work item)
+		std::size_t const slots = std::size_t( std::max( { _grid_update, _grid_flush, v.ntiles * v.rparts } ) ) * NSTAT;
 		if( slots > _bstats.size() )
 		{
 			std::vector<unsigned long long> keep( NSTAT, 0ull );
@@ -779,6 +786,8 @@ private:
 	{
 		if( _lc.recv_block == 256 )
 			SPICE_LAUNCH( k_receive_tiled<Model, F, 256><<<_grid_tiles, 256, 0, _s>>>( v ) );
+		else if( _lc.recv_block == 512 )
+			SPICE_LAUNCH( k_receive_tiled<Model, F, 512><<<_grid_tiles, 512, 0, _s>>>( v ) );
 		else
 			SPICE_LAUNCH( k_receive_tiled<Model, F, 1024><<<_grid_tiles, 1024, 0, _s>>>( v ) );
 	}

@@ -240,19 +240,21 @@ class Snn:
     # ---- launch shapes ----
     @staticmethod
     def _launch_dict(c):
-        return {"tile_w": c.tile_w, "flush_slots": c.flush_slots, "flush_grid": c.flush_grid, "recv_block": c.recv_block}
+        return {"tile_w": c.tile_w, "flush_slots": c.flush_slots, "flush_grid": c.flush_grid, "recv_block": c.recv_block,
+                "recv_parts": c.recv_parts}
 
     def launch(self):
         """Current launch shapes: tile_w (targets per delivery tile), flush_slots (target slots
         per plasticity-flush work item), flush_grid (flush workgroups), recv_block (threads per
-        delivery workgroup)."""
+        delivery workgroup), recv_parts (delivery workgroups per tile)."""
         c = _lib.Launch()
         _lib.check(_lib.lib().spice_hip_get_launch(self._h, _c.byref(c)))
         return self._launch_dict(c)
 
-    def set_launch(self, tile_w=0, flush_slots=0, flush_grid=0, recv_block=0):
-        """Sets launch shapes (0 = default); results do not depend on them."""
-        c = _lib.Launch(int(tile_w), int(flush_slots), int(flush_grid), int(recv_block))
+    def set_launch(self, tile_w=0, flush_slots=0, flush_grid=0, recv_block=0, recv_parts=0):
+        """Sets launch shapes (0 = default); results do not depend on them (in fp32 mode the
+        summation order of delivered inputs may change, as it does between runs)."""
+        c = _lib.Launch(int(tile_w), int(flush_slots), int(flush_grid), int(recv_block), int(recv_parts))
         _lib.check(_lib.lib().spice_hip_set_launch(self._h, _c.byref(c)))
 
     def autotune(self, warmup=500, steps=100):

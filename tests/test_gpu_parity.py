@@ -181,12 +181,12 @@ def test_launch_shapes_do_not_change_results(gpu):
     lay = three_pop_layout(2500)
     a = Snn("brunel_3pop_stdp", Layout(*lay), 1e-4, 15, seed=21, fixed_point=True)
     b = Snn("brunel_3pop_stdp", Layout(*lay), 1e-4, 15, seed=21, fixed_point=True)
-    shapes = [(64, 64, 1, 256), (1024, 2048, 4096, 1024), (320, 128, 37, 256), (0, 0, 0, 0)]
-    for k, (tw, fs, fg, bs) in enumerate(shapes):
-        b.set_launch(tw, fs, fg, bs)
+    shapes = [(64, 64, 1, 256, 1), (1024, 2048, 4096, 1024, 3), (320, 128, 37, 512, 16), (0, 0, 0, 0, 0)]
+    for k, (tw, fs, fg, bs, rp) in enumerate(shapes):
+        b.set_launch(tw, fs, fg, bs, rp)
         got = b.launch()
         if tw:
-            assert got == {"tile_w": tw, "flush_slots": fs, "flush_grid": fg, "recv_block": bs}
+            assert got == {"tile_w": tw, "flush_slots": fs, "flush_grid": fg, "recv_block": bs, "recv_parts": rp}
         a.run(400)
         b.run(400)
         assert_state_equal(b.neurons(), a.neurons(), f"shape {k}")
