@@ -290,7 +290,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2_000)
     ap.add_argument("--workload", default="brunel_stdp_1e9", choices=sorted(WORKLOADS))
     ap.add_argument("--profile-steps", type=int, default=1_000)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU baseline (0 = skip); ~0.55 s each at 1e9 synapses")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--flags", type=int, default=0, help="spice_flag launch-shape bits (see include/spice_hip.h)")
     ap.add_argument("--batch-nets", type=int, default=512, help="nets of the batched sweep (0 = skip)")
