@@ -95,7 +95,7 @@ public:
 	~hip_batch() override
 	{
 		if( _exec ) (void)hipGraphExecDestroy( _exec );
-		if( _lane.s ) (void)hipStreamSynchronize( _lane.s );
+		_lane.synchronize_quiet();
 		if( _s ) (void)hipStreamSynchronize( _s );
 		if( _s && _own_stream ) (void)hipStreamDestroy( _s );
 	}
@@ -132,7 +132,7 @@ public:
 
 	void synchronize() override
 	{
-		if( _lane.s ) SPICE_HIP( hipStreamSynchronize( _lane.s ) );
+		_lane.synchronize();
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 	int size() const override { return int( _nets.size() ); }
@@ -183,10 +183,11 @@ private:
 		SPICE_LAUNCH( k_batch_update<Model, F><<<_nblocks, 256, 0, _s>>>( v, _block_net.data(), _block_local.data() ) );
 		if constexpr( plastic )
 		{
+			hipStream_t const ls = _lane.s[t & 1];
 			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
-			SPICE_HIP( hipStreamWaitEvent( _lane.s, _lane.up, 0 ) );
-			SPICE_LAUNCH( k_batch_flush<Model, F><<<_nflush, 256, 0, _lane.s>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
-			SPICE_HIP( hipEventRecord( _lane.done[t & 1], _lane.s ) );
+			SPICE_HIP( hipStreamWaitEvent( ls, _lane.up, 0 ) );
+			SPICE_LAUNCH( k_batch_flush<Model, F><<<_nflush, 256, 0, ls>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
+			SPICE_HIP( hipEventRecord( _lane.done[t & 1], ls ) );
 		}
 	}
 

@@ -863,8 +863,8 @@ __global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
 				__syncthreads();
 			}
 		}
-	block_add_stats<3>(
-	    v, blockIdx.x, { touched, slots, blockIdx.x == 0 && tid == 0 ? std::uint64_t( nrows ) : 0ull }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	block_add_stats<3>( v, blockIdx.x + ( T & 1 ) * gridDim.x, { touched, slots, blockIdx.x == 0 && tid == 0 ? std::uint64_t( nrows ) : 0ull },
+	                    { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
 template <typename Model, bool FIXED>
@@ -1151,7 +1151,8 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 			if( p == 0 ) rows++;
 		}
 	}
-	block_add_stats<3>( v, blk, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	// the flushes of two consecutive steps may run at once: one set of slots per step parity
+	block_add_stats<3>( v, blk + ( T & 1 ) * nblk, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
 template <typename Model, bool FIXED>

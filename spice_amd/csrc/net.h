@@ -124,7 +124,7 @@ public:
 	~hip_net() override
 	{
 		if( _exec ) (void)hipGraphExecDestroy( _exec );
-		if( _lane.s ) (void)hipStreamSynchronize( _lane.s );
+		_lane.synchronize_quiet();
 		if( _s ) (void)hipStreamSynchronize( _s );
 		if( _s ) (void)hipStreamDestroy( _s );
 	}
@@ -237,7 +237,7 @@ public:
 
 	void synchronize() override
 	{
-		if( _lane.s ) SPICE_HIP( hipStreamSynchronize( _lane.s ) );
+		_lane.synchronize();
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
@@ -707,7 +707,8 @@ private:
 		_grid_tiles = std::min( v.ntiles * v.rparts, ( 16 * 1024 / c.recv_block ) * cus );
 		_lc = c;
 		// statistics slots: one per workgroup of the widest step kernel (receive: per work item)
-		std::size_t const slots = std::size_t( std::max( { _grid_update, _grid_flush, v.ntiles * v.rparts } ) ) * NSTAT;
+		// (flush: two sets, the flushes of consecutive steps may overlap; tiled flush: per tile)
+		std::size_t const slots = std::size_t( std::max( { _grid_update, 2 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts } ) ) * NSTAT;
 		if( slots > _bstats.size() )
 		{
 			std::vector<unsigned long long> keep( NSTAT, 0ull );
@@ -771,13 +772,14 @@ private:
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		if constexpr( plastic )
 		{
+			hipStream_t const ls = _lane.s[t & 1];
 			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
-			SPICE_HIP( hipStreamWaitEvent( _lane.s, _lane.up, 0 ) );
+			SPICE_HIP( hipStreamWaitEvent( ls, _lane.up, 0 ) );
 			if( _p.flags & SPICE_FLAG_FLUSH_TILED )
-				SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, _lane.s>>>( v ) );
+				SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, ls>>>( v ) );
 			else
-				SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _lane.s>>>( v ) );
-			SPICE_HIP( hipEventRecord( _lane.done[t & 1], _lane.s ) );
+				SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, ls>>>( v ) );
+			SPICE_HIP( hipEventRecord( _lane.done[t & 1], ls ) );
 		}
 	}
 
