@@ -200,20 +200,31 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
     back per chunk) vs the same jobs run one at a time as plain networks (a sample: every
     seq_stride-th job). Latency = submit -> last spikes delivered."""
     from spice_amd import Layout, Server, Snn
+    from spice_amd.raster import Raster
     jobs = serve_stream(count, rate, seed)
+    sample = list(range(0, count, seq_stride))
     streamed = [0]
+    rasters = {}  # job id -> Raster of the sampled jobs (rate / ISI CV vs the job run alone)
 
     def sink(job, first, off, ids):
         streamed[0] += len(ids)
+        r = rasters.get(job)
+        if r is not None:
+            r.add(first, off, ids)
 
     sv = Server(budget_bytes=budget, chunk_steps=chunk, on_spikes=sink)
     t0 = time.perf_counter()
     ids = []
-    for (t, m, pops, conns, d, steps, js) in jobs:
+    for k, (t, m, pops, conns, d, steps, js) in enumerate(jobs):
         lag = t - (time.perf_counter() - t0)
         if lag > 0:
             time.sleep(lag)
-        ids.append(sv.submit(m, Layout(pops, conns), 1e-4, steps, delay=d, seed=js))
+        lay = Layout(pops, conns)
+        if k in sample:
+            # registered before submit: the worker may deliver the first chunk at once
+            rasters[len(ids) + 1] = Raster(sum(pops), pops)
+        ids.append(sv.submit(m, lay, 1e-4, steps, delay=d, seed=js))
+        assert k not in sample or ids[-1] in rasters
     recs = [sv.wait(i) for i in ids]
     makespan = time.perf_counter() - t0
     info = sv.info()
@@ -245,6 +256,18 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
     seq = seq_events / seq_run
     seq_incl = seq_events / (seq_run + seq_setup)
     served = events / makespan
+    # statistics of the sampled jobs served in the mix vs the same jobs served alone
+    stat = []
+    for k in sample:
+        _, m, pops, conns, d, steps, js = jobs[k]
+        alone = Raster(sum(pops), pops)
+        with Server(budget_bytes=budget, chunk_steps=chunk,
+                    on_spikes=lambda job, first, off, ids_: alone.add(first, off, ids_)) as one:
+            one.wait(one.submit(m, Layout(pops, conns), 1e-4, steps, delay=d, seed=js))
+        mixed = rasters[ids[k]]
+        ra, rm = float(alone.count.sum()), float(mixed.count.sum())
+        cva, cvm = float(np.nanmean(alone.isi_cv())), float(np.nanmean(mixed.isi_cv()))
+        stat.append((ra, rm, cva, cvm))
     return {
         "jobs": count, "arrival_rate_per_s": rate, "budget_bytes": budget, "chunk_steps": chunk,
         "mix": "40% brunel_3pop / 40% vogels_abbott / 20% brunel_3pop_stdp, synapses log-uniform 1e5-2e9, "
@@ -263,6 +286,11 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
         "not_done": len(failed),
         "spike_count_rel_diff_total": round(abs(sum(a for a, _ in rel) - sum(b for _, b in rel))
                                             / max(1, sum(a for a, _ in rel)), 5),
+        "statistics_sample": f"{len(stat)} jobs served in the mix vs the same jobs served alone (fp32)",
+        "rate_rel_diff_pooled": round(abs(sum(x[1] for x in stat) - sum(x[0] for x in stat)) / max(1.0, sum(x[0] for x in stat)), 5),
+        "isi_cv_rel_diff_pooled": round(abs(np.nanmean([x[3] for x in stat]) - np.nanmean([x[2] for x in stat]))
+                                        / max(1e-9, abs(np.nanmean([x[2] for x in stat]))), 5),
+        "rate_rel_diff_max": round(max(abs(x[1] - x[0]) / max(1.0, x[0]) for x in stat), 5),
     }
 
 
@@ -359,6 +387,8 @@ def main():
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
         "bytes_per_launch": round(bytes_per[kern]), "avg_launch_us": round(avg_ms * 1e3, 3),
         "kernel_us_per_step": {k: round(v / args.profile_steps * 1e3, 3) for k, v in ms.items()},
+        # delivery accumulates every delivered event with one LDS atomic add per receive target
+        "receive_lds_atomics_per_s": round(pst["events"] / (ms["receive"] * 1e-3), 1) if ms.get("receive") else None,
     }
 
     cpu = None
