@@ -215,3 +215,22 @@ def test_autotune_keeps_the_network_in_place(gpu):
     c = Snn("brunel_3pop_stdp", Layout(*lay), 1e-4, 15, seed=5, fixed_point=True)
     cfg2, us2 = c.autotune(warmup=300, steps=50)
     assert cfg2 == cfg and us2 == us  # cached
+
+
+@pytest.mark.parametrize("delay", [1, 15])
+def test_trace_stdp_graph_replay_bit_exact(gpu, delay):
+    """HIP-graph replay of plastic steps (the flush of each step overlapping the next steps on
+    two side streams) equals the dense checker bit for bit over 1,500 steps (several flush
+    windows), with autotuned-style launch shapes (split delivery lists, small flush items)."""
+    model = "brunel_3pop_stdp"
+    lay = three_pop_layout(2000)
+    h = O.CpuSnn(model, lay, 1e-4, delay, seed=13, fixed_point=True)
+    d = Snn(model, Layout(*lay), 1e-4, delay, seed=13, fixed_point=True)
+    d.set_launch(256, 64, 64, 256, 3)
+    for k in range(3):
+        h.run(500)
+        d.run(500)  # 500 = 15 graphs of 32 steps + eager steps
+        assert_state_equal(d.neurons(), h.neurons(), f"after {500 * (k + 1)} steps")
+    assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), "weights")
+    st = d.stats()
+    assert st["plastic_events"] > 0 and st["flush_rows"] > 0 and st["flush_synapses"] > 0
