@@ -39,12 +39,32 @@ def three_pop(n, p):
 
 
 WORKLOADS = {
-    # name: (model, n, p, delay)
+    # name: (model, n, p, delay); brunel* models: three populations (Poisson / E / I), every
+    # pair at p; vogels: one population at p
     "brunel_stdp_1e9": ("brunel_3pop_stdp", 100_000, 0.1, 15),
     "brunel_1e9": ("brunel_3pop", 100_000, 0.1, 15),
     "vogels_1e8": ("vogels_abbott", 70_710, 0.02, 1),
     "brunel_1e10": ("brunel_3pop", 316_228, 0.1, 15),
+    "vogels_1e9": ("vogels_abbott", 223_607, 0.02, 1),
+    # the reference's own benchmark networks (samples/main.cpp:1265-1283, 1389-1402):
+    # brunel / brunel+: pops {N/2, N/2}, 0->1 and 1->1 at p = 0.1, N = sqrt(S / 0.05), delay 15;
+    # vogels: one population at p = 0.02, N = sqrt(S / 0.02), delay 8
+    "ref_brunel_1e9": ("brunel", 141_421, 0.1, 15),
+    "ref_vogels_1e9": ("vogels_abbott", 223_606, 0.02, 8),
+    "ref_brunel_plus_6e8": ("brunel_with_plasticity", 109_544, 0.1, 15),
 }
+# wall/bio the reference publishes for those networks on one GPU (results/spice.json:218-225,
+# 122-129, 378-385; GPU model not stated)
+PUBLISHED = {"ref_brunel_1e9": 0.98772, "ref_vogels_1e9": 1.08669, "ref_brunel_plus_6e8": 21.8024}
+
+
+def workload_layout(name):
+    model, n, p, delay = WORKLOADS[name]
+    if name.startswith("ref_brunel"):
+        return [n // 2, n // 2], [(0, 1, p), (1, 1, p)]
+    if "brunel" in model:
+        return three_pop(n, p)
+    return [n], [(0, 0, p)]
 
 
 def algorithmic_bytes(model, n, st, steps):
@@ -342,7 +362,7 @@ def main():
     from spice_amd import Layout, Snn
 
     model, n, p, delay = WORKLOADS[args.workload]
-    pops, conns = three_pop(n, p) if "brunel" in model else ([n], [(0, 0, p)])
+    pops, conns = workload_layout(args.workload)
     dt = 1e-4
 
     t0 = time.perf_counter()
@@ -415,7 +435,7 @@ def main():
         "ms_per_step": round(wall / args.steps * 1e3, 5),
         "higher_is_better": False,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": round(wall / bio / PUBLISHED[args.workload], 5) if args.workload in PUBLISHED else None,
         "dtype": "f32",
         "data": "synthetic (device-generated random graph, fixed seed)",
         "config": {"workload": args.workload, "model": model, "neurons": n, "synapses": E,
