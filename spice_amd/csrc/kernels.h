@@ -39,7 +39,7 @@ constexpr int HWORDS = HISTORY / 64;     // 64-bit words per neuron in the histo
 // history; the generic contract merges an event-free stretch into one update(n) whose decay
 // pow(d, n) differs from n per-step products by up to ~n/2 ulp, so its stretches stay short.
 template <typename S>
-constexpr int flush_age_v = has_neuron_traces_v<S> ? HISTORY - 1 : 255;
+constexpr int flush_age_v = has_neuron_traces_v<S> || HISTORY - 1 < 255 ? HISTORY - 1 : 255;
 
 template <typename T>
 __device__ inline T ld( void const * base, std::int64_t i )
