@@ -1,0 +1,53 @@
+"""The bench's synthetic workloads match their configuration text: network sizes, topology,
+the served stream's model mix, size and duration distributions, the batched sweep."""
+import math
+
+import numpy as np
+
+import bench
+
+
+def _synapses(pops, conns):
+    return sum(pops[s] * pops[d] * p for s, d, p in conns)
+
+
+def test_headline_and_reference_networks():
+    pops, conns = bench.workload_layout("brunel_stdp_1e9")
+    assert pops == [20_000, 64_000, 16_000] and len(conns) == 9 and all(p == 0.1 for *_, p in conns)
+    assert abs(_synapses(pops, conns) - 1e9) < 1e6
+    pops, conns = bench.workload_layout("ref_brunel_1e9")  # samples/main.cpp: {N/2, N/2}, 0->1, 1->1
+    assert pops == [70_710, 70_710] and conns == [(0, 1, 0.1), (1, 1, 0.1)]
+    assert abs(_synapses(pops, conns) - 1e9) / 1e9 < 1e-3
+    pops, conns = bench.workload_layout("ref_vogels_1e9")
+    assert pops == [223_606] and abs(_synapses(pops, conns) - 1e9) / 1e9 < 1e-3
+    assert bench.WORKLOADS["ref_vogels_1e9"][3] == 8 and bench.WORKLOADS["ref_brunel_1e9"][3] == 15
+    pops, conns = bench.workload_layout("brunel_1e10")
+    assert abs(_synapses(pops, conns) - 1e10) / 1e10 < 1e-3
+    assert set(bench.PUBLISHED) <= set(bench.WORKLOADS)
+
+
+def test_served_stream_distribution():
+    jobs = bench.serve_stream(4000, 2.0, 3)
+    models = [j[1] for j in jobs]
+    frac = {m: models.count(m) / len(jobs) for m in set(models)}
+    assert abs(frac["brunel_3pop"] - 0.4) < 0.03 and abs(frac["vogels_abbott"] - 0.4) < 0.03
+    assert abs(frac["brunel_3pop_stdp"] - 0.2) < 0.03
+    syn = np.array([_synapses(j[2], j[3]) for j in jobs])
+    assert syn.min() >= 0.9e5 and syn.max() <= 2.1e9
+    lg = np.log(syn)  # log-uniform: uniform in log space
+    assert abs(np.mean(lg) - (math.log(1e5) + math.log(2e9)) / 2) < 0.2
+    bio = np.array([j[5] for j in jobs]) * 1e-4
+    assert bio.min() >= 0.1 - 1e-4 and bio.max() <= 10 + 1e-4 and abs(bio.mean() - 5.05) < 0.3
+    gaps = np.diff([0.0] + [j[0] for j in jobs])
+    assert abs(gaps.mean() - 0.5) < 0.05  # Poisson arrivals at 2 jobs/s
+    assert all(j[4] == (1 if j[1] == "vogels_abbott" else 15) for j in jobs)
+
+
+def test_batched_sweep():
+    nets = bench.sweep_nets(512, 7)
+    assert sum(1 for n in nets if n[0] == "brunel_3pop") == 256
+    syn = np.array([_synapses(n[1], n[2]) for n in nets])
+    assert syn.min() > 0.8e4 and syn.max() < 1.2e8
+    rates = [n[5][0] for n in nets if n[0] == "brunel_3pop"]
+    assert min(rates) >= 5 and max(rates) <= 30
+    assert len({n[4] for n in nets}) == 512  # per-net seeds
