@@ -48,6 +48,23 @@ public:
 		_fixed = _nets[0]->fixed();
 		for( auto * n : _nets )
 			if( n->fixed() != _fixed ) throw std::invalid_argument( "all nets of a batch use the same accumulation mode" );
+		// Many nets already fill the GPU, so then a batched net does not need many small
+		// delivery tiles: each net gets about max(64, 64 x CUs / nets) tiles (fewer workgroups,
+		// each with a fixed staging cost). On a sweep of two 256-net batches 64 tiles per net
+		// beat 16, 32, 128 and 256; a batch of a few nets keeps its nets' own tiles. Fixed-point
+		// results do not depend on the tile width (fp32 sums are reordered, as by any launch
+		// shape); a net keeps wider tiles it already has, and keeps the new width afterwards.
+		std::int64_t const tiles_per_net = std::max<std::int64_t>( 64, std::int64_t( SPICE_BATCH_ITEMS_PER_CU ) * device_cu_count() / std::int64_t( _nets.size() ) );
+		for( auto * n : _nets )
+		{
+			launch_cfg c = n->get_launch();
+			int const tw = int( std::clamp<std::int64_t>( ( n->num_neurons() / tiles_per_net + 63 ) / 64 * 64, 64, TILE_MAX ) );
+			if( tw > c.tile_w )
+			{
+				c.tile_w = tw;
+				n->set_launch( c );
+			}
+		}
 		std::vector<std::int32_t> tn, tl, bn, bl, fn, fl, fc;
 		std::vector<dview> vs[CLOCK_SLOTS];
 		for( std::size_t k = 0; k < _nets.size(); k++ )

@@ -444,6 +444,9 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 #ifndef SPICE_RECV_UNROLL
 #define SPICE_RECV_UNROLL 2
 #endif
+#ifndef SPICE_BATCH_ITEMS_PER_CU
+#define SPICE_BATCH_ITEMS_PER_CU 64
+#endif
 #ifndef SPICE_BATCH_RECV_BLOCK
 #define SPICE_BATCH_RECV_BLOCK 256
 #endif
