@@ -76,11 +76,15 @@ def algorithmic_bytes(model, n, st, steps):
                plastic synapse changed, the write-back of its state (4 for neuron-trace
                synapses, 12 + 8 re-read for per-synapse-trace synapses);
       update : neuron state read + write, accumulator read + clear, plus for plastic models
-               history word read + write (16), row bookkeeping (9)
-               and, for neuron-trace synapses, traces (16) and trace rings (8) per neuron;
+               history word read + write (16) and row bookkeeping (9 lazy: synced, pending
+               event, has-plastic flag; 1 eager: the flag), for neuron-trace synapses (eager)
+               the traces read + written (16) and the step's flags (1) per neuron;
                4 per emitted spike;
-      flush  : target ids of every scanned slot of the flushed rows (4) + per changed synapse
-               its state read + write (8 neuron-trace, 24 per-synapse-trace).
+      flush  : the side-stream plasticity pass: 4 per index word read (lazy flush: the target id
+               of every scanned slot of the flushed rows; eager: the target id of every plastic
+               slot of the arriving rows and the (source, offset) pair of every by-target entry
+               of the spiking targets) + per changed synapse its state read + write (8
+               neuron-trace, 24 per-synapse-trace).
     Target-neuron accumulation is counted as atomics, not bytes."""
     from spice_amd.models import MODELS
     _, nattrs, sattrs = MODELS[model]
@@ -94,7 +98,7 @@ def algorithmic_bytes(model, n, st, steps):
     pev = st["plastic_events"] / steps
     wb = 4 if traced else (2 * s_s - 4)
     recv = spk * 20 + ev * (8 if plastic else 4) + pev * (wb if plastic else 0)
-    upd = n * (2 * s_n + 2 * acc_bytes + (25 if plastic else 0) + (24 if traced else 0)) + spk * 4
+    upd = n * (2 * s_n + 2 * acc_bytes + ((17 + 17) if traced else (25 if plastic else 0))) + spk * 4
     fl = (st["flush_slots"] * 4 + st["flush_synapses"] * (2 * s_s)) / steps if plastic else 0.0
     return {"receive": recv, "update": upd, "flush": fl}
 

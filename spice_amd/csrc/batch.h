@@ -181,24 +181,42 @@ private:
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
-	// Same schedule as hip_net::launch_step: the flush of step t overlaps step t + 1.
-	void launch_step( std::int64_t t, bool head = false )
+	// Same schedule as hip_net::launch_step: the side work of step t overlaps step t + 1.
+	void launch_step( std::int64_t t, int fresh = 2 )
 	{
 		if( _fixed )
-			launch_step_t<true>( t, head );
+			launch_step_t<true>( t, fresh );
 		else
-			launch_step_t<false>( t, head );
+			launch_step_t<false>( t, fresh );
 	}
 
 	template <bool F>
-	void launch_step_t( std::int64_t t, bool head )
+	void launch_step_t( std::int64_t t, int fresh )
 	{
+		constexpr bool traced = has_neuron_traces_v<typename Model::synapse>;
 		dview const * v = _views[t & ( CLOCK_SLOTS - 1 )].data();
 		if constexpr( plastic )
-			if( !head ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
+			if( fresh >= 2 )
+			{
+				SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
+				if constexpr( traced ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done2[t & 1], 0 ) );
+			}
 		SPICE_LAUNCH( k_batch_receive<Model, F, recv_block><<<_grid_tiles, recv_block, 0, _s>>>( v, _tile_net.data(), _tile_local.data(), _ntiles ) );
 		SPICE_LAUNCH( k_batch_update<Model, F><<<_nblocks, 256, 0, _s>>>( v, _block_net.data(), _block_local.data() ) );
-		if constexpr( plastic )
+		if constexpr( traced )
+		{
+			// same lane rule as hip_net::launch_step_t: pre passes on s[0], post passes on s[1]
+			// after the previous step's pre pass
+			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
+			SPICE_HIP( hipStreamWaitEvent( _lane.s[0], _lane.up, 0 ) );
+			SPICE_LAUNCH( k_batch_stdp<Model, F, false><<<_nflush, 256, 0, _lane.s[0]>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
+			SPICE_HIP( hipStreamWaitEvent( _lane.s[1], _lane.up, 0 ) );
+			if( fresh >= 1 ) SPICE_HIP( hipStreamWaitEvent( _lane.s[1], _lane.done[( t - 1 ) & 1], 0 ) );
+			SPICE_HIP( hipEventRecord( _lane.done[t & 1], _lane.s[0] ) );
+			SPICE_LAUNCH( k_batch_stdp<Model, F, true><<<_nflush, 256, 0, _lane.s[1]>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
+			SPICE_HIP( hipEventRecord( _lane.done2[t & 1], _lane.s[1] ) );
+		}
+		else if constexpr( plastic )
 		{
 			hipStream_t const ls = _lane.s[t & 1];
 			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
@@ -214,7 +232,7 @@ private:
 		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
 		try
 		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step( i, i < 2 );
+			for( int i = 0; i < graph_steps; i++ ) launch_step( i, i );
 			_lane.join_into( _s );
 		}
 		catch( ... )

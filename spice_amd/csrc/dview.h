@@ -12,17 +12,23 @@
 //   ring     int32[(D+1) * N], counts uint32[D+1] spike ids of the last D+1 steps
 //   hring    uint64[HWORDS * N]  spike-history ring: bit (t mod H) of word (t mod H)/64 of
 //                        neuron i set iff i spiked at step t (H = 256 steps, plastic models)
+//   hasp     uint8[N]    row s has at least one plastic synapse
+//   pl_lo/hi int32[N]    first / one-past-last plastic slot of row s (offsets in the row)
+//   pivots   int32[N * (ntiles+1)]  per row and target tile: offset of the tile's first target
+// lazy plasticity (per-synapse contract):
 //   last_spk int32[N]    latest spike step
 //   synced   int32[N]    row s is up to date through this step
 //   pend     int32[N]    row s has an undelivered pre event at synced+1
-//   hasp     uint8[N]    row s has at least one plastic synapse
-//   pl_lo/hi int32[N]    first / one-past-last plastic slot of row s (offsets in the row)
-//   flush_*  int32[2][N] rows whose history window is full this step, with their
+//   fl_*     int32[2][N] rows whose history window is full this step, with their
 //                        (synced, pend) at that time; lists by step parity, counters [4]
 //                        by step mod 4
-//   pivots   int32[N * (ntiles+1)]  per row and target tile: offset of the tile's first target
-//   xtr/ytr  f32[N]      neuron-local pre/post traces (trace synapses)
-//   xh/yh    f32[H * N]  their values over the last H steps (ring, slot = step mod H)
+// eager plasticity (neuron-trace synapses), all [2][N] arrays by step parity:
+//   xtr/ytr  f32[2][N]   neuron-local pre/post traces after step T
+//   fl8      u8[2][N]    bit 0: spiked at T, bit 1: its spike arrives at T, bit 2: at T + 1
+//   fl_id    int32[2][N] rows whose spike arrived at T (and does not arrive at T + 1)
+//   po_id    int32[2][N] neurons that spiked at T and have incoming plastic synapses
+//   tp_ptr   int64[N+1], tp_ent {src, offset in src's row}[P]: the P plastic synapses by target
+//   caught   int64       step whose deferred events (rows arriving next) a sync already applied
 //   slog_ids int32[cap]  spike log (served jobs): ids of steps t0 .. t0+nsteps-1, step by step
 //   slog_off u64[nsteps] offset of step t0+k in slog_ids
 #include <cstdint>
@@ -67,11 +73,19 @@ struct slog_t
 	std::uint64_t cap;
 };
 
+// One plastic synapse of the by-target index: its source row and its offset in that row.
+struct alignas( 8 ) tp_entry
+{
+	std::int32_t src;
+	std::int32_t off;
+};
+
 struct dview
 {
 	std::int32_t N, D, width, shift;
 	float DT;
 	std::int32_t fparts; // work items per flushed row
+	std::int32_t pparts; // work items per spiking target (eager plasticity)
 	std::int32_t rparts; // delivery workgroups per tile (the step's spike list split between them)
 	std::uint32_t flags;
 	std::uint64_t seed;
@@ -100,8 +114,12 @@ struct dview
 	std::uint32_t * fl_count; // [4]
 	float * xtr;
 	float * ytr;
-	float * xh;
-	float * yh;
+	std::uint8_t * fl8;
+	std::int32_t * po_id;
+	std::uint32_t * po_count; // [4]
+	std::int64_t const * tp_ptr;
+	tp_entry const * tp_ent;
+	std::int64_t * caught;
 
 	ctl_t * ctl;      // this step's clock
 	ctl_t * ctl_next; // the next step's clock (written by k_update)

@@ -7,12 +7,14 @@
 //                  reads) and calls M::neuron::receive for each target through a proxy
 //                  iterator that routes backend.atomic_add into per-attribute accumulators
 //                  (fp32 atomics, or int64 fixed point). Plastic models first bring each
-//                  plastic synapse up to date (lazy STDP replay) and then receive reads it.
+//                  plastic synapse up to date (lazy replay, or the deferred events of the
+//                  previous step for neuron-trace synapses) and then receive reads it.
 //   k_update<M>  : one thread per neuron: add accumulated inputs, run M::neuron::update,
 //                  store, and append spikes to the ring with one atomic per wave
-//                  (ballot + popcount + mbcnt). Plastic models also shift the per-neuron
-//                  spike history and queue rows whose history window is full.
-//   k_flush<M>   : plastic models only: bring queued rows up to date.
+//                  (ballot + popcount + mbcnt). Plastic models also record the spike
+//                  history and queue rows (and, eagerly, spiking targets) for k_flush.
+//   k_flush<M>   : plastic models only: bring queued rows up to date (lazy), or apply this
+//                  step's pre and post events (neuron-trace synapses, eager).
 // The last kernel of a step advances the device clock (last-block-done), so the same
 // launches are valid at every step and can be captured once into a HIP graph.
 //
@@ -32,14 +34,13 @@
 
 namespace spice::hipx
 {
-constexpr int HISTORY = 256;            // spike-history window (steps) of lazy plasticity
+constexpr int HISTORY = 256;            // spike-history window (steps) of plastic models
 constexpr int HWORDS = HISTORY / 64;     // 64-bit words per neuron in the history ring
-// Rows are brought up to date at least every flush_age_v<Synapse> steps. Trace synapses replay
-// events only (bit-exact with per-step updates for any window), so they use the whole
-// history; the generic contract merges an event-free stretch into one update(n) whose decay
-// pow(d, n) differs from n per-step products by up to ~n/2 ulp, so its stretches stay short.
-template <typename S>
-constexpr int flush_age_v = has_neuron_traces_v<S> || HISTORY - 1 < 255 ? HISTORY - 1 : 255;
+// Lazy plasticity (per-synapse contract): rows are brought up to date at least every
+// FLUSH_AGE steps, before their window leaves the history. An event-free stretch is merged
+// into one update(n) whose decay pow(d, n) differs from n per-step products by up to ~n/2
+// ulp, which also keeps the stretches this short.
+constexpr int FLUSH_AGE = HISTORY - 1;
 
 template <typename T>
 __device__ inline T ld( void const * base, std::int64_t i )
@@ -292,105 +293,23 @@ __device__ inline void replay(
 	if( t < now ) Syn::update( syn, now - t, false, false, dt, info, bak );
 }
 
-// Neuron-local-trace plasticity (trace_synapse models): the same event sequence, but an
-// event calls Syn::on_event with the source's pre trace and the target's post trace at the
-// event's step, read from the trace rings. Event-free steps change nothing by contract, so a
-// synapse whose target did not spike in the window (and with no pending pre event) is not
-// touched at all. Returns whether the synapse changed.
-template <typename Syn, typename Iter, typename Bak>
-__device__ inline bool replay_traced(
-    Iter syn, dview const & v, std::int32_t s, std::int32_t j, std::int32_t now, std::int32_t synced, std::int32_t pend, snn_info info, Bak & bak )
-{
-	if( now <= synced ) return false;
-	std::int32_t const ls = v.last_spk[j];
-	if( !pend && ls <= synced ) return false;
-	std::int64_t const N = v.N;
-	std::int32_t first = synced + 1;
-	if( pend )
-	{
-		bool const post = ls >= first && spiked_at( v, j, first );
-		std::int64_t const slot = first & ( HISTORY - 1 );
-		Syn::on_event( syn, true, post, post ? v.xh[slot * N + s] : 0.0f, v.yh[slot * N + j], info, bak );
-		first++;
-	}
-	for_each_spike( v, j, first, ls < now ? ls : now, [&]( std::int32_t e ) {
-		std::int64_t const slot = e & ( HISTORY - 1 );
-		Syn::on_event( syn, false, true, v.xh[slot * N + s], 0.0f, info, bak ); // post only: y is not read
-	} );
-	return true;
-}
+// ---- eager plasticity (neuron-trace synapses) ----------------------------------------------
+// Trace synapses change only at events, and an event needs only the two neurons' traces of
+// that step, so every event is applied in (or right after) the step it happens, and the
+// synapse state is current at all times. Step T's event of synapse s -> d (pre: s's spike
+// arrives at T, post: d spiked at T; one on_event call with both flags) is applied by exactly
+// one of:
+//   * the pre pass of step T: rows whose spike arrives at T and not at T + 1;
+//   * the post pass of step T: for d spiked at T, its incoming plastic synapses (by-target
+//     index) whose source's spike arrives neither at T nor at T + 1;
+//   * the delivery of step T + 1 (deferred): rows whose spike arrives at T + 1, before the
+//     delivery reads them.
+// The two passes run as one kernel on a side stream while step T + 1 delivers and updates:
+// they touch no row that delivery reads, they read only step T's traces and flags (arrays by
+// step parity), and consecutive steps' passes run in order on one stream.
 
-// Bit e (a step within the window) of a neuron's history words.
-__device__ inline bool hbit( hwords const & h, std::int32_t e )
-{
-	int const idx = e & ( HISTORY - 1 );
-	unsigned long long word = 0;
-#pragma unroll
-	for( int w = 0; w < HWORDS; w++ )
-		if( w == ( idx >> 6 ) ) word = h.w[w];
-	return ( word >> ( idx & 63 ) ) & 1ull;
-}
-
-// Everything replay_traced reads that does not depend on the event list, loaded ahead for
-// several synapses at once: the target's latest spike and history words (if it spiked after
-// synced) and, for a pending pre event, the target's post trace at that step (the source's
-// pre trace at that step, xf, is one value per row).
-struct traced_pre
-{
-	hwords h;
-	float yf;
-	std::int32_t ls;
-};
-
-__device__ inline traced_pre load_traced( dview const & v, std::int32_t j, std::int32_t ls, std::int32_t synced, std::int32_t pend )
-{
-	traced_pre t;
-	t.ls = ls;
-	if( ls > synced )
-		t.h = load_hwords( v, j );
-	else
-	{
-#pragma unroll
-		for( int w = 0; w < HWORDS; w++ ) t.h.w[w] = 0ull;
-	}
-	t.yf = pend ? v.yh[std::int64_t( ( synced + 1 ) & ( HISTORY - 1 ) ) * v.N + j] : 0.0f;
-	return t;
-}
-
-// replay_traced over preloaded data (same event sequence and arithmetic): only the traces at
-// the target's post spikes in the window are loaded here.
-// Returns whether any event was applied (a target whose latest spike is after `now` -- it
-// spiked in a step running concurrently -- may have none in the window).
-template <typename Syn, typename Iter, typename Bak>
-__device__ inline bool replay_traced_pre(
-    Iter syn, dview const & v, std::int32_t s, std::int32_t j, std::int32_t now, std::int32_t synced, std::int32_t pend, float xf,
-    traced_pre const & t, snn_info info, Bak & bak )
-{
-	std::int64_t const N = v.N;
-	std::int32_t first = synced + 1;
-	bool any = false;
-	if( pend )
-	{
-		bool const post = t.ls >= first && hbit( t.h, first );
-		Syn::on_event( syn, true, post, post ? xf : 0.0f, t.yf, info, bak );
-		first++;
-		any = true;
-	}
-	std::int32_t const last = t.ls < now ? t.ls : now;
-	if( first <= last )
-		for_each_spike( t.h, first, last, [&]( std::int32_t e ) {
-			std::int64_t const slot = e & ( HISTORY - 1 );
-			Syn::on_event( syn, false, true, v.xh[slot * N + s], 0.0f, info, bak ); // post only: y is not read
-			any = true;
-		} );
-	return any;
-}
-
-// Whether synapse src -> j has an event in (synced, now] (trace synapses only need those).
-__device__ inline bool has_event( dview const & v, std::int32_t j, std::int32_t synced, std::int32_t pend )
-{
-	return pend || v.last_spk[j] > synced;
-}
+// flag bits of fl8
+constexpr std::uint8_t F8_SPIKED = 1, F8_ARRIVED = 2, F8_ARRIVES_NEXT = 4;
 
 // ---- kernels -------------------------------------------------------------------------------
 
@@ -584,9 +503,10 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	__shared__ std::int64_t seg_beg[BS];
 	__shared__ std::int32_t seg_pre[BS + 1];
 	__shared__ std::int32_t seg_src[BS];
-	__shared__ std::int32_t seg_synced[plastic ? BS : 1];
-	__shared__ std::int32_t seg_pend[plastic ? BS : 1];
-	__shared__ float seg_xf[plastic && has_neuron_traces_v<S> ? BS : 1];
+	constexpr bool eager = plastic && has_neuron_traces_v<S>;
+	__shared__ std::int32_t seg_synced[plastic && !eager ? BS : 1];
+	__shared__ std::int32_t seg_pend[plastic ? BS : 1]; // eager: the row's spike also arrived at T - 1
+	__shared__ float seg_xf[eager ? BS : 1];            // eager: the row's pre trace after T - 1
 	__shared__ std::int32_t wsum[BS / 64];
 
 	std::int64_t const T = v.ctl->step;
@@ -607,6 +527,10 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	snn_info const info = info_of( v );
 	int const tid = threadIdx.x;
 	unsigned long long ev = 0, pev = 0;
+	// eager: step T - 1's events of the delivered rows were left to this delivery, unless a
+	// synchronization (state export) has applied them already
+	bool const catch_up = eager && T >= 1 && *v.caught != T - 1;
+	std::int64_t const prev = ( ( T - 1 ) & 1 ) * std::int64_t( v.N ); // step T - 1's arrays
 	{
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
@@ -625,13 +549,15 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 				seg_beg[tid] = v.row_ptr[s] + a;
 				seg_src[tid] = s;
 				len = b - a;
-				if constexpr( plastic )
+				if constexpr( eager )
 				{
-					std::int32_t const sy = v.synced[s], pd = v.pend[s];
-					seg_synced[tid] = sy;
-					seg_pend[tid] = pd;
-					if constexpr( has_neuron_traces_v<S> ) // the source's pre trace at its pending event
-						seg_xf[tid] = pd ? v.xh[std::int64_t( ( sy + 1 ) & ( HISTORY - 1 ) ) * v.N + s] : 0.0f;
+					seg_pend[tid] = catch_up && ( v.fl8[prev + s] & F8_ARRIVED );
+					seg_xf[tid] = catch_up ? v.xtr[prev + s] : 0.0f;
+				}
+				else if constexpr( plastic )
+				{
+					seg_synced[tid] = v.synced[s];
+					seg_pend[tid] = v.pend[s];
 				}
 			}
 			// exclusive block scan of the segment lengths
@@ -656,29 +582,24 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					e[u] = seg_beg[a] + ( f - seg_pre[a] );
 					j[u] = ok[u] ? v.col[e[u]] : 0;
 				}
-				if constexpr( plastic && has_neuron_traces_v<S> )
+				if constexpr( eager )
 				{
-					// trace synapses: load every synapse's state and replay inputs first (independent
-					// loads for all u), then replay the events and deliver
+					// neuron-trace synapses: step T - 1's event of each plastic synapse (the row's
+					// spike arrived at T - 1, or the target spiked at T - 1), then delivery. The
+					// state, the target's flags and (for a pre event) its post trace are loaded
+					// for all u before any event is applied.
 					typename S::tuple_t sy[U];
-					traced_pre tp[U];
-					bool ev[U];
-					std::int32_t ls[U];
+					std::uint8_t f8[U];
+					float yv[U];
 #pragma unroll
 					for( int u = 0; u < U; u++ )
 						if( ok[u] ) load_tuple<S>( sy[u], v.sa, e[u] );
 #pragma unroll
 					for( int u = 0; u < U; u++ )
 					{
-						ev[u] = ok[u] && S::plastic( seg_src[kk[u]], j[u], info );
-						ls[u] = ev[u] ? v.last_spk[j[u]] : -1;
-					}
-#pragma unroll
-					for( int u = 0; u < U; u++ )
-					{
-						std::int32_t const synced = seg_synced[kk[u]], pend = seg_pend[kk[u]];
-						ev[u] = ev[u] && std::int32_t( T - 1 ) > synced && ( pend || ls[u] > synced );
-						if( ev[u] ) tp[u] = load_traced( v, j[u], ls[u], synced, pend );
+						bool const pl = ok[u] && catch_up && S::plastic( seg_src[kk[u]], j[u], info );
+						f8[u] = pl ? std::uint8_t( v.fl8[prev + j[u]] | ( seg_pend[kk[u]] ? 0x80 : 0 ) ) : std::uint8_t( 0 );
+						yv[u] = pl && seg_pend[kk[u]] ? v.ytr[prev + j[u]] : 0.0f;
 					}
 #pragma unroll
 					for( int u = 0; u < U; u++ )
@@ -688,9 +609,13 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
 						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc };
 						reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
-						if( ev[u] &&
-						    replay_traced_pre<S>( itr, v, s, j[u], std::int32_t( T - 1 ), seg_synced[kk[u]], seg_pend[kk[u]], seg_xf[kk[u]], tp[u], info, bak ) )
+						bool const pre = ( f8[u] & 0x80 ) != 0, post = ( f8[u] & F8_SPIKED ) != 0;
+						if( pre || post )
 						{
+							// the event of step T - 1, keyed like the pass that would have applied it
+							auto sbak = make_backend<FIXED>( v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ),
+							                                 std::uint32_t( T - 1 ), util::TAG_SYNAPSE_UPDATE );
+							S::on_event( itr, pre, post, post ? seg_xf[kk[u]] : 0.0f, yv[u], info, sbak );
 							store_tuple<S>( sy[u], v.sa, e[u] );
 							pev++;
 						}
@@ -777,14 +702,14 @@ __global__ __launch_bounds__( BS ) void k_batch_receive(
 
 
 
-// Brings the rows queued by this step's neuron update up to date (lazy plasticity), one
-// workgroup per target tile: the queued rows' plastic segments inside the tile are walked as
-// one flat index space, and only synapses with an event in their row's window are read and
-// written.
+// Brings the rows queued by this step's neuron update up to date (lazy plasticity of the
+// per-synapse contract), one workgroup per target tile: the queued rows' plastic segments
+// inside the tile are walked as one flat index space.
 template <typename Model, bool FIXED>
 __global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
 {
 	using S = typename Model::synapse;
+	static_assert( !has_neuron_traces_v<S>, "neuron-trace synapses use the eager passes" );
 	constexpr int U = 2;
 	__shared__ std::int64_t seg_beg[SPK_CHUNK];
 	__shared__ std::int32_t seg_pre[SPK_CHUNK + 1];
@@ -847,17 +772,12 @@ __global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
 						std::int32_t const s = seg_src[kk[u]];
 						if( !S::plastic( s, j[u], info ) ) continue;
 						std::int32_t const synced = seg_synced[kk[u]], pend = seg_pend[kk[u]];
-						if constexpr( has_neuron_traces_v<S> )
-							if( !has_event( v, j[u], synced, pend ) ) continue;
 						typename S::tuple_t sy;
 						load_tuple<S>( sy, v.sa, e[u] );
 						auto bak = make_backend<FIXED>(
 						    v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
 						reg_iter<typename S::tuple_t> itr{ sy, std::uint32_t( e[u] ) };
-						if constexpr( has_neuron_traces_v<S> )
-							replay_traced<S>( itr, v, s, j[u], std::int32_t( T ), synced, pend, info, bak );
-						else
-							replay<S>( itr, v, j[u], std::int32_t( T ), synced, pend, dt, info, bak );
+						replay<S>( itr, v, j[u], std::int32_t( T ), synced, pend, dt, info, bak );
 						store_tuple<S>( sy, v.sa, e[u] );
 						touched++;
 					}
@@ -874,7 +794,9 @@ template <typename Model, bool FIXED>
 __device__ inline void update_block( dview const & v, std::int64_t blk )
 {
 	using N = typename Model::neuron;
+	using S = typename Model::synapse;
 	constexpr bool plastic = is_plastic_v<Model>;
+	constexpr bool eager = plastic && has_neuron_traces_v<S>;
 	constexpr std::uint32_t rmask = receive_mask_v<N>;
 
 	std::int64_t const T = v.ctl->step;
@@ -884,6 +806,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 	{
 		v.counts[( T + 1 ) % D1] = 0; // consumed by this step's receive; refilled at T + 1
 		if constexpr( plastic ) v.fl_count[( T + 1 ) & 3] = 0; // step T - 1's flush may still read slot (T - 1) & 3
+		if constexpr( eager ) v.po_count[( T + 1 ) & 3] = 0;
 		// next step's clock: exactly kahan_sum::add on the host
 		ctl_t const c = *v.ctl;
 		float const y = v.DT - c.c;
@@ -895,7 +818,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
 	std::int64_t const i = blk * 256 + threadIdx.x;
-	bool spiked = false, flush = false;
+	bool spiked = false, flush = false, post = false;
 	std::int32_t fl_synced = 0, fl_pend = 0;
 
 	if( i < v.N )
@@ -928,90 +851,102 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 
 		if constexpr( plastic )
 		{
-			using S = typename Model::synapse;
 			std::int64_t const N64 = v.N;
-			std::int32_t synced = v.synced[i], pend = v.pend[i];
-			bool dirty = false;
 			// this neuron's spike of step T - D reaches its synapses now
 			bool const arrived = T >= v.D && spiked_at( v, std::int32_t( i ), std::int32_t( T - v.D ) );
-			if( arrived )
-			{
-				synced = std::int32_t( T - 1 );
-				pend = 1;
-				dirty = true;
-			}
+			// and its spike of step T + 1 - D at the next step (recorded already unless D = 1)
+			bool const arrives_next = T + 1 >= v.D && ( v.D == 1 ? spiked : spiked_at( v, std::int32_t( i ), std::int32_t( T + 1 - v.D ) ) );
 			{ // record this step in the history ring (overwriting step T - HISTORY)
 				std::int64_t const w = ( T & ( HISTORY - 1 ) ) >> 6;
 				unsigned long long const bit = 1ull << ( T & 63 );
 				unsigned long long const word = v.hring[w * N64 + i];
 				unsigned long long const nw = spiked ? ( word | bit ) : ( word & ~bit );
 				if( nw != word ) v.hring[w * N64 + i] = nw;
-				if( spiked ) v.last_spk[i] = std::int32_t( T );
 			}
-			if constexpr( has_neuron_traces_v<S> )
+			if constexpr( eager )
 			{
+				// traces after step T (the dense checker's per-step recurrence) and the step's
+				// flags, into the arrays of step parity T & 1
+				std::int64_t const cur = ( T & 1 ) * N64, prv = N64 - cur;
 				float const d = S::trace_decay( dt );
-				float const x = fmaf( v.xtr[i], d, arrived ? 1.0f : 0.0f );
-				float const y = fmaf( v.ytr[i], d, spiked ? 1.0f : 0.0f );
-				v.xtr[i] = x;
-				v.ytr[i] = y;
-				std::int64_t const slot = T & ( HISTORY - 1 );
-				v.xh[slot * N64 + i] = x;
-				v.yh[slot * N64 + i] = y;
+				v.xtr[cur + i] = fmaf( v.xtr[prv + i], d, arrived ? 1.0f : 0.0f );
+				v.ytr[cur + i] = fmaf( v.ytr[prv + i], d, spiked ? 1.0f : 0.0f );
+				v.fl8[cur + i] = std::uint8_t( ( spiked ? F8_SPIKED : 0 ) | ( arrived ? F8_ARRIVED : 0 ) | ( arrives_next ? F8_ARRIVES_NEXT : 0 ) );
+				// pre pass: a row whose spike arrives now, unless it arrives again at T + 1 (then
+				// that delivery applies this step's events of the row); post pass: a spiking
+				// neuron with incoming plastic synapses
+				flush = arrived && !arrives_next && v.hasp[i];
+				post = spiked && v.tp_ptr[i + 1] > v.tp_ptr[i];
 			}
-			// Rows are flushed before their window overflows; a row never processed yet gets its
-			// first flush at step (i mod age), which spreads flushes evenly over steps.
-			constexpr int age = flush_age_v<S>;
-			// Rows go to this step's flush list (and are current through T afterwards) when
-			//  * their spike arrived now: the pending pre event of step T is applied right away,
-			//    while every such row reads the same, cache-resident trace slot of step T, or
-			//  * their history window is full.
-			// A row whose spike arrives at T + 1 is left to that delivery instead: the flush of
-			// step T may run while step T + 1 delivers, so their rows must differ.
-			bool const arrives_next = T + 1 >= v.D && ( v.D == 1 ? spiked : spiked_at( v, std::int32_t( i ), std::int32_t( T + 1 - v.D ) ) );
-			if( v.hasp[i] && !arrives_next && ( pend || T - synced >= age || ( synced < 0 && T >= i % age ) ) )
+			else
 			{
-				flush = true;
-				fl_synced = synced;
-				fl_pend = pend;
-				synced = std::int32_t( T );
-				pend = 0;
-				dirty = true;
-			}
-			if( dirty )
-			{
-				v.synced[i] = synced;
-				v.pend[i] = pend;
+				std::int32_t synced = v.synced[i], pend = v.pend[i];
+				bool dirty = false;
+				if( arrived )
+				{
+					synced = std::int32_t( T - 1 );
+					pend = 1;
+					dirty = true;
+				}
+				if( spiked ) v.last_spk[i] = std::int32_t( T );
+				// Rows are flushed before their window overflows; a row never processed yet gets
+				// its first flush at step (i mod age), which spreads flushes evenly over steps.
+				constexpr int age = FLUSH_AGE;
+				// Rows go to this step's flush list (and are current through T afterwards) when
+				//  * their spike arrived now: the pending pre event of step T is applied right
+				//    away, or
+				//  * their history window is full.
+				// A row whose spike arrives at T + 1 is left to that delivery instead: the flush
+				// of step T may run while step T + 1 delivers, so their rows must differ.
+				if( v.hasp[i] && !arrives_next && ( pend || T - synced >= age || ( synced < 0 && T >= i % age ) ) )
+				{
+					flush = true;
+					fl_synced = synced;
+					fl_pend = pend;
+					synced = std::int32_t( T );
+					pend = 0;
+					dirty = true;
+				}
+				if( dirty )
+				{
+					v.synced[i] = synced;
+					v.pend[i] = pend;
+				}
 			}
 		}
 	}
 
 	// Spike compaction: ballot + popcount per wave, prefix over the block's waves in LDS, one
 	// atomic per block on the step's counter, mbcnt gives each spiking lane its slot.
-	__shared__ unsigned wcount[4][2];
-	__shared__ unsigned bbase[2];
+	__shared__ unsigned wcount[4][3];
+	__shared__ unsigned bbase[3];
 	int const w = threadIdx.x >> 6;
 	std::uint64_t const m = ballot( spiked );
 	std::uint64_t const fm = plastic ? ballot( flush ) : 0ull;
+	std::uint64_t const pm = eager ? ballot( post ) : 0ull;
 	if( lane_id() == 0 )
 	{
 		wcount[w][0] = unsigned( __popcll( m ) );
 		wcount[w][1] = unsigned( __popcll( fm ) );
+		wcount[w][2] = unsigned( __popcll( pm ) );
 	}
 	__syncthreads();
 	if( threadIdx.x == 0 )
 	{
-		unsigned t0 = 0, t1 = 0;
+		unsigned t0 = 0, t1 = 0, t2 = 0;
 		for( int k = 0; k < 4; k++ )
 		{
-			unsigned const a = wcount[k][0], b = wcount[k][1];
+			unsigned const a = wcount[k][0], b = wcount[k][1], c = wcount[k][2];
 			wcount[k][0] = t0;
 			wcount[k][1] = t1;
+			wcount[k][2] = t2;
 			t0 += a;
 			t1 += b;
+			t2 += c;
 		}
 		bbase[0] = t0 ? atomicAdd( &v.counts[wslot], t0 ) : 0u;
 		if constexpr( plastic ) bbase[1] = t1 ? atomicAdd( &v.fl_count[T & 3], t1 ) : 0u;
+		if constexpr( eager ) bbase[2] = t2 ? atomicAdd( &v.po_count[T & 3], t2 ) : 0u;
 		if( t0 ) v.bstats[blk * NSTAT + ST_SPIKES] += t0;
 	}
 	__syncthreads();
@@ -1036,9 +971,16 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 		{
 			std::int64_t const o = ( T & 1 ) * std::int64_t( v.N ) + bbase[1] + wcount[w][1] + lanes_below( fm );
 			v.fl_id[o] = std::int32_t( i );
-			v.fl_synced[o] = fl_synced;
-			v.fl_pend[o] = fl_pend;
+			if constexpr( !eager )
+			{
+				v.fl_synced[o] = fl_synced;
+				v.fl_pend[o] = fl_pend;
+			}
 		}
+	}
+	if constexpr( eager )
+	{
+		if( post ) v.po_id[( T & 1 ) * std::int64_t( v.N ) + bbase[2] + wcount[w][2] + lanes_below( pm )] = std::int32_t( i );
 	}
 }
 
@@ -1055,12 +997,177 @@ __global__ __launch_bounds__( 256 ) void k_batch_update( dview const * views, st
 	update_block<Model, FIXED>( views[block_net[blockIdx.x]], block_local[blockIdx.x] );
 }
 
-// Row-major flush of the rows whose history window is full: work item = (row, part); waves
-// of workgroups blk, blk + nblk, ... (of this net) take items round robin.
+// Eager plasticity of step T (neuron-trace synapses): the pre pass and the post pass as one
+// index space of work items, (row, part) of the pre rows then (target, part) of the spiking
+// targets, taken round robin by the waves of workgroups blk, blk + nblk, ... (of this net).
+//   pre  item: a slice of the row's plastic slots; every plastic synapse gets step T's event
+//              (pre, post = target spiked at T) with the row's pre trace and the target's post
+//              trace after step T.
+//   post item: a slice of the target's incoming plastic synapses (by-target index); a
+//              synapse whose source's spike arrives at T or T + 1 is left to the pre pass or
+//              to the next delivery, the others get a post event with the source's pre trace.
+// Both kinds write disjoint synapses (a synapse is in a pre row iff its source arrived at T).
+#ifndef SPICE_POST_UNROLL
+#define SPICE_POST_UNROLL 4
+#endif
+template <typename Model, bool FIXED, bool POST>
+__device__ inline void stdp_block( dview const & v, std::int64_t blk, std::int64_t nblk )
+{
+	using S = typename Model::synapse;
+	constexpr int U = 4;                  // pre pass: synapses per lane per iteration in flight
+	constexpr int UP = SPICE_POST_UNROLL; // post pass: by-target entries per lane per iteration
+	std::int64_t const T = v.ctl->step;
+	std::int64_t const cur = ( T & 1 ) * std::int64_t( v.N ); // step T's lists, traces and flags
+	std::int64_t const nitems = POST ? std::int64_t( v.po_count[T & 3] ) * v.pparts : std::int64_t( v.fl_count[T & 3] ) * v.fparts;
+	snn_info const info = info_of( v );
+	int const lane = lane_id();
+	std::int64_t const wave0 = uniform64( ( blk * blockDim.x + threadIdx.x ) >> 6 );
+	std::int64_t const nwaves = nblk * ( blockDim.x >> 6 );
+	unsigned long long touched = 0, words = 0, rows = 0;
+	for( std::int64_t it = wave0; it < nitems; it += nwaves )
+	{
+		if constexpr( !POST )
+		{
+			std::int64_t const f = it / v.fparts;
+			std::int64_t const p = it - f * v.fparts;
+			std::int32_t const s = uniform( v.fl_id[cur + f] );
+			std::int64_t const r0 = v.row_ptr[s];
+			std::int64_t const lo = r0 + v.pl_lo[s], hi = r0 + v.pl_hi[s];
+			std::int64_t const chunk = ( ( hi - lo + v.fparts - 1 ) / v.fparts + 63 ) & ~std::int64_t( 63 );
+			std::int64_t const beg = lo + p * chunk;
+			std::int64_t const end = beg + chunk < hi ? beg + chunk : hi;
+			float const xs = v.xtr[cur + s];
+			for( std::int64_t base = beg; base < end; base += 64 * U )
+			{
+				std::int32_t j[U];
+				bool ok[U];
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					std::int64_t const e = base + lane + 64 * u;
+					ok[u] = e < end;
+					j[u] = ok[u] ? v.col[e] : 0;
+				}
+				typename S::tuple_t sy[U];
+				std::uint8_t f8[U];
+				float yv[U];
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					ok[u] = ok[u] && S::plastic( s, j[u], info );
+					if( ok[u] )
+					{
+						load_tuple<S>( sy[u], v.sa, base + lane + 64 * u );
+						f8[u] = v.fl8[cur + j[u]];
+						yv[u] = v.ytr[cur + j[u]];
+					}
+				}
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					if( !ok[u] ) continue;
+					std::int64_t const e = base + lane + 64 * u;
+					auto bak = make_backend<FIXED>(
+					    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
+					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e ) };
+					bool const post = ( f8[u] & F8_SPIKED ) != 0;
+					S::on_event( itr, true, post, post ? xs : 0.0f, yv[u], info, bak );
+					store_tuple<S>( sy[u], v.sa, e );
+					touched++;
+				}
+			}
+			if( lane == 0 )
+			{
+				if( end > beg ) words += std::uint64_t( end - beg ); // target ids
+				if( p == 0 ) rows++;
+			}
+		}
+		else
+		{
+			std::int64_t const f = it / v.pparts;
+			std::int64_t const p = it - f * v.pparts;
+			std::int32_t const d = uniform( v.po_id[cur + f] );
+			std::int64_t const lo = v.tp_ptr[d], hi = v.tp_ptr[d + 1];
+			std::int64_t const chunk = ( ( hi - lo + v.pparts - 1 ) / v.pparts + 63 ) & ~std::int64_t( 63 );
+			std::int64_t const beg = lo + p * chunk;
+			std::int64_t const end = beg + chunk < hi ? beg + chunk : hi;
+			for( std::int64_t base = beg; base < end; base += 64 * UP )
+			{
+				tp_entry te[UP];
+				bool ok[UP];
+#pragma unroll
+				for( int u = 0; u < UP; u++ )
+				{
+					std::int64_t const k = base + lane + 64 * u;
+					ok[u] = k < end;
+					te[u] = ok[u] ? v.tp_ent[k] : tp_entry{ 0, 0 };
+				}
+				// the source's flags, row start and trace: independent loads, issued together
+				std::int64_t e[UP];
+				float xv[UP];
+				std::uint8_t f8[UP];
+#pragma unroll
+				for( int u = 0; u < UP; u++ )
+				{
+					f8[u] = v.fl8[cur + te[u].src];
+					e[u] = v.row_ptr[te[u].src] + te[u].off;
+					xv[u] = v.xtr[cur + te[u].src];
+				}
+#pragma unroll
+				for( int u = 0; u < UP; u++ ) ok[u] = ok[u] && !( f8[u] & ( F8_ARRIVED | F8_ARRIVES_NEXT ) );
+				typename S::tuple_t sy[UP];
+#pragma unroll
+				for( int u = 0; u < UP; u++ )
+					if( ok[u] ) load_tuple<S>( sy[u], v.sa, e[u] );
+#pragma unroll
+				for( int u = 0; u < UP; u++ )
+				{
+					if( !ok[u] ) continue;
+					auto bak = make_backend<FIXED>(
+					    v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
+					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
+					S::on_event( itr, false, true, xv[u], 0.0f, info, bak ); // post only: y is not read
+					store_tuple<S>( sy[u], v.sa, e[u] );
+					touched++;
+				}
+			}
+			if( lane == 0 )
+			{
+				if( end > beg ) words += 2 * std::uint64_t( end - beg ); // (source, offset) entries
+				if( p == 0 ) rows++;
+			}
+		}
+	}
+	// statistics slots: pre and post passes of the same or adjacent steps may run at once, so
+	// one set per (pass, step parity)
+	block_add_stats<3>( v, blk + ( ( POST ? 2 : 0 ) + ( T & 1 ) ) * nblk, { touched, words, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+}
+
+// Eager plasticity of step T: the pre pass (POST = false) or the post pass (POST = true).
+template <typename Model, bool FIXED, bool POST>
+__global__ __launch_bounds__( 256 ) void k_stdp( dview v )
+{
+	stdp_block<Model, FIXED, POST>( v, blockIdx.x, gridDim.x );
+}
+
+// Batched eager pass: workgroup b is block block_local[b] of the block_count[b] blocks of net
+// block_net[b].
+template <typename Model, bool FIXED, bool POST>
+__global__ __launch_bounds__( 256 ) void k_batch_stdp(
+    dview const * views, std::int32_t const * block_net, std::int32_t const * block_local, std::int32_t const * block_count )
+{
+	std::int32_t const b = blockIdx.x;
+	stdp_block<Model, FIXED, POST>( views[block_net[b]], block_local[b], block_count[b] );
+}
+
+// Row-major flush of the rows whose history window is full (lazy plasticity of the
+// per-synapse contract): work item = (row, part); waves of workgroups blk, blk + nblk, ...
+// (of this net) take items round robin.
 template <typename Model, bool FIXED>
 __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int64_t nblk )
 {
 	using S = typename Model::synapse;
+	static_assert( !has_neuron_traces_v<S>, "neuron-trace synapses use the eager passes" );
 	constexpr int U = 4; // slots per lane per iteration: independent loads in flight
 	std::int64_t const T = v.ctl->step;
 	std::int64_t const nitems = std::int64_t( v.fl_count[T & 3] ) * v.fparts;
@@ -1083,8 +1190,6 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 		std::int64_t const chunk = ( ( hi - lo + v.fparts - 1 ) / v.fparts + 63 ) & ~std::int64_t( 63 );
 		std::int64_t const beg = lo + p * chunk;
 		std::int64_t const end = beg + chunk < hi ? beg + chunk : hi;
-		// the source's pre trace at its pending event (one value per row)
-		float const xf = has_neuron_traces_v<S> && pend ? v.xh[std::int64_t( ( synced + 1 ) & ( HISTORY - 1 ) ) * v.N + s] : 0.0f;
 		for( std::int64_t base = beg; base < end; base += 64 * U )
 		{
 			std::int32_t j[U];
@@ -1098,41 +1203,6 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 			}
 #pragma unroll
 			for( int u = 0; u < U; u++ ) ok[u] = ok[u] && S::plastic( s, j[u], info );
-			std::int32_t ls[U];
-#pragma unroll
-			for( int u = 0; u < U; u++ ) ls[u] = ok[u] ? v.last_spk[j[u]] : -1;
-			if constexpr( has_neuron_traces_v<S> )
-			{
-				// events only: load the state of every synapse with an event first, then replay
-				bool ev[U];
-				typename S::tuple_t sy[U];
-				traced_pre tp[U];
-#pragma unroll
-				for( int u = 0; u < U; u++ )
-				{
-					ev[u] = ok[u] && ( pend || ls[u] > synced );
-					if( ev[u] )
-					{
-						load_tuple<S>( sy[u], v.sa, base + lane + 64 * u );
-						tp[u] = load_traced( v, j[u], ls[u], synced, pend );
-					}
-				}
-#pragma unroll
-				for( int u = 0; u < U; u++ )
-				{
-					if( !ev[u] ) continue;
-					std::int64_t const e = base + lane + 64 * u;
-					auto bak = make_backend<FIXED>(
-					    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
-					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e ) };
-					if( replay_traced_pre<S>( itr, v, s, j[u], std::int32_t( T ), synced, pend, xf, tp[u], info, bak ) )
-					{
-						store_tuple<S>( sy[u], v.sa, e );
-						touched++;
-					}
-				}
-				continue;
-			}
 #pragma unroll
 			for( int u = 0; u < U; u++ )
 			{
@@ -1187,11 +1257,12 @@ static __global__ __launch_bounds__( 256 ) void k_batch_log_window( dview const 
 }
 
 // Brings every row with plastic synapses up to date through step `now` (= last completed
-// step); one wave per row. Used before exporting synapse state.
+// step); one wave per row. Used before exporting synapse state (lazy plasticity).
 template <typename Model>
 __global__ __launch_bounds__( 256 ) void k_sync_all( dview v, std::int32_t now, float dt )
 {
 	using S = typename Model::synapse;
+	static_assert( !has_neuron_traces_v<S>, "neuron-trace synapses are synchronized by k_sync_eager" );
 	std::int64_t const r = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
 	if( r >= v.N || !v.hasp[r] ) return;
 	snn_info const info = info_of( v );
@@ -1205,20 +1276,118 @@ __global__ __launch_bounds__( 256 ) void k_sync_all( dview v, std::int32_t now, 
 		load_tuple<S>( sy, v.sa, e );
 		auto bak = make_backend<false>( v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( now ), util::TAG_SYNAPSE_UPDATE );
 		reg_iter<typename S::tuple_t> it{ sy, std::uint32_t( e ) };
-		if constexpr( has_neuron_traces_v<S> )
-		{
-			if( replay_traced<S>( it, v, std::int32_t( r ), j, now, synced, pend, info, bak ) ) store_tuple<S>( sy, v.sa, e );
-		}
-		else
-		{
-			replay<S>( it, v, j, now, synced, pend, dt, info, bak );
-			store_tuple<S>( sy, v.sa, e );
-		}
+		replay<S>( it, v, j, now, synced, pend, dt, info, bak );
+		store_tuple<S>( sy, v.sa, e );
 	}
 	if( lane_id() == 0 )
 	{
 		v.synced[r] = now;
 		v.pend[r] = 0;
+	}
+}
+
+// Eager plasticity: applies the events of step `now` (= last completed step) that were left
+// to the next delivery -- those of the rows whose spike arrives at now + 1 -- exactly as that
+// delivery would; one wave per row of the arriving list. The host then records `now` in
+// v.caught so the delivery skips them. Used before exporting synapse state.
+template <typename Model>
+__global__ __launch_bounds__( 256 ) void k_sync_eager( dview v, std::int64_t now )
+{
+	using S = typename Model::synapse;
+	std::int64_t const x = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
+	std::int64_t const T = now + 1; // the step whose delivery would apply them
+	if( T < v.D ) return;
+	int const D1 = v.D + 1;
+	if( x >= std::int64_t( v.counts[( T + 1 ) % D1] ) ) return; // spikes of step T - D
+	std::int32_t const r = v.ring[std::int64_t( ( T + 1 ) % D1 ) * v.N + x];
+	if( !v.hasp[r] ) return;
+	snn_info const info = info_of( v );
+	std::int64_t const prev = ( now & 1 ) * std::int64_t( v.N );
+	bool const pre = ( v.fl8[prev + r] & F8_ARRIVED ) != 0;
+	float const xs = v.xtr[prev + r];
+	std::int64_t const r0 = v.row_ptr[r];
+	for( std::int64_t e = r0 + v.pl_lo[r] + lane_id(); e < r0 + v.pl_hi[r]; e += 64 )
+	{
+		std::int32_t const j = v.col[e];
+		if( !S::plastic( r, j, info ) ) continue;
+		bool const post = ( v.fl8[prev + j] & F8_SPIKED ) != 0;
+		if( !pre && !post ) continue;
+		typename S::tuple_t sy;
+		load_tuple<S>( sy, v.sa, e );
+		auto bak = make_backend<false>( v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( now ), util::TAG_SYNAPSE_UPDATE );
+		reg_iter<typename S::tuple_t> it{ sy, std::uint32_t( e ) };
+		S::on_event( it, pre, post, post ? xs : 0.0f, pre ? v.ytr[prev + j] : 0.0f, info, bak );
+		store_tuple<S>( sy, v.sa, e );
+	}
+}
+
+// ---- by-target index of the plastic synapses (eager plasticity) ----------------------------
+// Counts every target's incoming plastic synapses; one wave per source row.
+template <typename Model>
+__global__ __launch_bounds__( 256 ) void k_tp_count( dview v, std::uint32_t * cnt )
+{
+	using S = typename Model::synapse;
+	std::int64_t const r = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
+	if( r >= v.N ) return;
+	snn_info const info = info_of( v );
+	for( std::int64_t e = v.row_ptr[r] + lane_id(); e < v.row_ptr[r + 1]; e += 64 )
+	{
+		std::int32_t const j = v.col[e];
+		if( S::plastic( std::int32_t( r ), j, info ) ) atomicAdd( &cnt[j], 1u );
+	}
+}
+
+// ptr[0..n] = exclusive prefix sums of cnt[0..n-1]; one workgroup of 1024 threads, each
+// summing a contiguous slice (n is a neuron count: this runs once per network).
+static inline __global__ __launch_bounds__( 1024 ) void k_tp_scan( std::uint32_t const * cnt, std::int64_t * ptr, std::int64_t n )
+{
+	__shared__ std::int64_t part[1024];
+	int const t = threadIdx.x;
+	std::int64_t const per = ( n + 1023 ) / 1024;
+	std::int64_t const a = t * per < n ? t * per : n, b = a + per < n ? a + per : n;
+	std::int64_t sum = 0;
+	for( std::int64_t i = a; i < b; i++ ) sum += cnt[i];
+	part[t] = sum;
+	__syncthreads();
+	if( t == 0 )
+	{
+		std::int64_t run = 0;
+		for( int k = 0; k < 1024; k++ )
+		{
+			std::int64_t const x = part[k];
+			part[k] = run;
+			run += x;
+		}
+		ptr[n] = run;
+	}
+	__syncthreads();
+	std::int64_t run = part[t];
+	for( std::int64_t i = a; i < b; i++ )
+	{
+		ptr[i] = run;
+		run += cnt[i];
+	}
+}
+
+// Fills the index: every plastic synapse appends (source, offset in the source's row) to its
+// target's list (cursor: a copy of ptr). The order inside a list is arbitrary: each entry is
+// an independent synapse.
+template <typename Model>
+__global__ __launch_bounds__( 256 ) void k_tp_fill( dview v, unsigned long long * cursor, tp_entry * ent )
+{
+	using S = typename Model::synapse;
+	std::int64_t const r = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
+	if( r >= v.N ) return;
+	snn_info const info = info_of( v );
+	std::int64_t const r0 = v.row_ptr[r];
+	for( std::int64_t e = r0 + lane_id(); e < v.row_ptr[r + 1]; e += 64 )
+	{
+		std::int32_t const j = v.col[e];
+		if( S::plastic( std::int32_t( r ), j, info ) )
+		{
+			unsigned long long const k = atomicAdd( &cursor[j], 1ull );
+			ent[k] = tp_entry{ std::int32_t( r ), std::int32_t( e - r0 ) };
+		}
 	}
 }
 } // namespace spice::hipx

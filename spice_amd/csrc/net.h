@@ -266,13 +266,16 @@ public:
 		if constexpr( plastic )
 		{
 			SPICE_HIP( hipMemset( _hring.data(), 0, _hring.bytes() ) );
-			SPICE_HIP( hipMemset( _last_spk.data(), 0xff, _last_spk.bytes() ) );
-			std::vector<std::int32_t> synced( _g.n, std::int32_t( step - 1 ) );
-			SPICE_HIP( hipMemcpy( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice ) );
-			SPICE_HIP( hipMemset( _pend.data(), 0, _pend.bytes() ) );
 			SPICE_HIP( hipMemset( _fl_count.data(), 0, _fl_count.bytes() ) );
 			if constexpr( traced )
-				for( auto * b : { &_xtr, &_ytr, &_xh, &_yh } ) SPICE_HIP( hipMemset( b->data(), 0, b->bytes() ) );
+				reset_eager();
+			else
+			{
+				SPICE_HIP( hipMemset( _last_spk.data(), 0xff, _last_spk.bytes() ) );
+				std::vector<std::int32_t> synced( _g.n, std::int32_t( step - 1 ) );
+				SPICE_HIP( hipMemcpy( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice ) );
+				SPICE_HIP( hipMemset( _pend.data(), 0, _pend.bytes() ) );
+			}
 		}
 		_step = step;
 		_clock.set( sum, c );
@@ -438,7 +441,8 @@ public:
 		for( int a = 0; a < MAXA; a++ ) b += _na[a].bytes() + _acc[a].bytes() + _sa[a].bytes();
 		b += _ring.bytes() + _counts.bytes() + _hring.bytes() + _last_spk.bytes() + _synced.bytes() + _pend.bytes() +
 		     _fl_id.bytes() + _fl_synced.bytes() + _fl_pend.bytes() + _hasp.bytes() + _pl_lo.bytes() + _pl_hi.bytes() +
-		     _fl_count.bytes() + _xtr.bytes() + _ytr.bytes() + _xh.bytes() + _yh.bytes() + _ctl.bytes() + _bstats.bytes() +
+		     _fl_count.bytes() + _xtr.bytes() + _ytr.bytes() + _fl8.bytes() + _po_id.bytes() + _po_count.bytes() +
+		     _tp_ptr.bytes() + _caught.bytes() + _tp_ent.bytes() + _ctl.bytes() + _bstats.bytes() +
 		     _pivots.bytes() + _slog_ctl.bytes() + _slog_ids.bytes() + _slog_off.bytes();
 		return b;
 	}
@@ -492,11 +496,11 @@ public:
 	int num_receive_parts() const { return _v.rparts; }
 	int num_update_blocks() const { return _grid_update; }
 	// flush workgroups of this net inside a batch: about two per expected work item wave
-	// (rows reach their flush about every flush_age_v steps, and once after each arrival of
-	// their spike), at most the net's own grid
+	// (lazy: rows reach their flush about every FLUSH_AGE steps, and once after each arrival
+	// of their spike; eager: arriving rows and spiking targets), at most the net's own grid
 	int num_batch_flush_blocks() const
 	{
-		std::int64_t const items = ( _g.n / flush_age_v<S> + _g.n / 256 + 1 ) * _v.fparts;
+		std::int64_t const items = traced ? ( _g.n / 256 + 1 ) * ( _v.fparts + _v.pparts ) : ( _g.n / FLUSH_AGE + _g.n / 256 + 1 ) * _v.fparts;
 		return int( std::clamp<std::int64_t>( ( items + 1 ) / 2, 1, _grid_flush ) );
 	}
 	bool fixed() const { return ( _p.flags & SPICE_FLAG_FIXED_POINT ) != 0; }
@@ -526,7 +530,14 @@ private:
 	dbuf<std::uint8_t> _hasp;
 	dbuf<std::int32_t> _pl_lo, _pl_hi;
 	dbuf<std::uint32_t> _fl_count;
-	dbuf<float> _xtr, _ytr, _xh, _yh;
+	dbuf<float> _xtr, _ytr;                    // eager: [2][N] traces by step parity
+	dbuf<std::uint8_t> _fl8;                   // eager: [2][N] step flags
+	dbuf<std::int32_t> _po_id;                 // eager: [2][N] spiking targets
+	dbuf<std::uint32_t> _po_count;             // eager: [4]
+	dbuf<std::int64_t> _tp_ptr, _caught;       // eager: by-target index offsets; synced step
+	dbuf<tp_entry> _tp_ent;                    // eager: by-target index entries
+	std::int64_t _caught_host = -1;
+	std::int64_t _tp_total = 0, _tp_targets = 0;
 	dbuf<ctl_t> _ctl;
 	dbuf<unsigned long long> _bstats;
 	std::int64_t _stats_step0 = 0;
@@ -582,31 +593,37 @@ private:
 		if constexpr( plastic )
 		{
 			_hring.resize( std::size_t( HWORDS ) * n );
-			_last_spk.resize( n );
-			_synced.resize( n );
-			_pend.resize( n );
 			_hasp.resize( n );
 			_pl_lo.resize( n );
 			_pl_hi.resize( n );
 			_fl_id.resize( 2 * n ); // lists of steps of both parities
-			_fl_synced.resize( 2 * n );
-			_fl_pend.resize( 2 * n );
 			_fl_count.resize( CLOCK_SLOTS );
 			SPICE_HIP( hipMemsetAsync( _hring.data(), 0, _hring.bytes(), _s ) );
-			SPICE_HIP( hipMemsetAsync( _last_spk.data(), 0xff, _last_spk.bytes(), _s ) ); // -1
-			SPICE_HIP( hipMemsetAsync( _pend.data(), 0, _pend.bytes(), _s ) );
 			SPICE_HIP( hipMemsetAsync( _fl_count.data(), 0, _fl_count.bytes(), _s ) );
 			if constexpr( traced )
 			{
-				_xtr.resize( n );
-				_ytr.resize( n );
-				_xh.resize( std::size_t( HISTORY ) * n );
-				_yh.resize( std::size_t( HISTORY ) * n );
-				for( auto * b : { &_xtr, &_ytr, &_xh, &_yh } ) SPICE_HIP( hipMemsetAsync( b->data(), 0, b->bytes(), _s ) );
+				_xtr.resize( 2 * n );
+				_ytr.resize( 2 * n );
+				_fl8.resize( 2 * n );
+				_po_id.resize( 2 * n );
+				_po_count.resize( CLOCK_SLOTS );
+				_caught.resize( 1 );
+				_tp_ptr.resize( n + 1 );
+				reset_eager();
 			}
-			std::vector<std::int32_t> synced( n, -1 );
-			SPICE_HIP( hipMemcpyAsync( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice, _s ) );
-			SPICE_HIP( hipStreamSynchronize( _s ) ); // `synced` is a host temporary
+			else
+			{
+				_last_spk.resize( n );
+				_synced.resize( n );
+				_pend.resize( n );
+				_fl_synced.resize( 2 * n );
+				_fl_pend.resize( 2 * n );
+				SPICE_HIP( hipMemsetAsync( _last_spk.data(), 0xff, _last_spk.bytes(), _s ) ); // -1
+				SPICE_HIP( hipMemsetAsync( _pend.data(), 0, _pend.bytes(), _s ) );
+				std::vector<std::int32_t> synced( n, -1 );
+				SPICE_HIP( hipMemcpyAsync( _synced.data(), synced.data(), _synced.bytes(), hipMemcpyHostToDevice, _s ) );
+				SPICE_HIP( hipStreamSynchronize( _s ) ); // `synced` is a host temporary
+			}
 		}
 		_ctl.resize( CLOCK_SLOTS );
 		SPICE_HIP( hipMemsetAsync( _ctl.data(), 0, _ctl.bytes(), _s ) );
@@ -643,13 +660,59 @@ private:
 		v.fl_count = _fl_count.data();
 		v.xtr = _xtr.data();
 		v.ytr = _ytr.data();
-		v.xh = _xh.data();
-		v.yh = _yh.data();
+		v.fl8 = _fl8.data();
+		v.po_id = _po_id.data();
+		v.po_count = _po_count.data();
+		v.tp_ptr = _tp_ptr.data();
+		v.caught = _caught.data();
 		v.ctl = _ctl.data();
 		v.ctl_next = _ctl.data() + 1;
 
+		if constexpr( traced ) build_by_target();
 		_grid_update = int( ( n + 255 ) / 256 );
 		apply_launch( launch_cfg{} );
+	}
+
+	// Eager plasticity state at a clock (re)start: zero traces and flags, no pending lists,
+	// nothing synchronized.
+	void reset_eager()
+	{
+		for( auto * b : { &_xtr, &_ytr } ) SPICE_HIP( hipMemsetAsync( b->data(), 0, b->bytes(), _s ) );
+		SPICE_HIP( hipMemsetAsync( _fl8.data(), 0, _fl8.bytes(), _s ) );
+		SPICE_HIP( hipMemsetAsync( _po_count.data(), 0, _po_count.bytes(), _s ) );
+		std::int64_t const none = -1;
+		SPICE_HIP( hipMemcpyAsync( _caught.data(), &none, 8, hipMemcpyHostToDevice, _s ) );
+		SPICE_HIP( hipStreamSynchronize( _s ) ); // `none` is a host temporary
+		_caught_host = -1;
+	}
+
+	// By-target index of the plastic synapses: count per target, prefix sums, fill.
+	void build_by_target()
+	{
+		std::int64_t const n = _g.n;
+		unsigned const rows_grid = unsigned( ( n + 3 ) / 4 ); // one wave per row
+		dbuf<std::uint32_t> cnt( n );
+		SPICE_HIP( hipMemsetAsync( cnt.data(), 0, cnt.bytes(), _s ) );
+		SPICE_LAUNCH( k_tp_count<Model><<<rows_grid, 256, 0, _s>>>( _v, cnt.data() ) );
+		SPICE_LAUNCH( k_tp_scan<<<1, 1024, 0, _s>>>( cnt.data(), _tp_ptr.data(), n ) );
+		std::int64_t total = 0;
+		SPICE_HIP( hipMemcpyAsync( &total, _tp_ptr.data() + n, 8, hipMemcpyDeviceToHost, _s ) );
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+		_tp_total = total;
+		_tp_ent.resize( std::size_t( std::max<std::int64_t>( total, 1 ) ) );
+		_v.tp_ent = _tp_ent.data();
+		if( total > 0 )
+		{
+			dbuf<unsigned long long> cursor( n );
+			SPICE_HIP( hipMemcpyAsync( cursor.data(), _tp_ptr.data(), 8 * n, hipMemcpyDeviceToDevice, _s ) );
+			SPICE_LAUNCH( k_tp_fill<Model><<<rows_grid, 256, 0, _s>>>( _v, cursor.data(), _tp_ent.data() ) );
+			SPICE_HIP( hipStreamSynchronize( _s ) ); // before the temporaries are freed
+		}
+		// targets with incoming plastic synapses (the mean in-degree sets the post-pass split)
+		std::vector<std::int64_t> ptr( n + 1 );
+		SPICE_HIP( hipMemcpy( ptr.data(), _tp_ptr.data(), ptr.size() * 8, hipMemcpyDeviceToHost ) );
+		_tp_targets = 0;
+		for( std::int64_t i = 0; i < n; i++ ) _tp_targets += ptr[i + 1] > ptr[i];
 	}
 
 	// Launch shapes (zero fields: the defaults). tile_w: targets per delivery tile (the
@@ -702,13 +765,16 @@ private:
 		}
 		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
 		v.fparts = std::max( 1, int( mean_row / c.flush_slots + 0.5 ) );
+		double const mean_in = _tp_targets > 0 ? double( _tp_total ) / double( _tp_targets ) : 0.0;
+		v.pparts = std::max( 1, int( mean_in / c.flush_slots + 0.5 ) );
 		v.rparts = c.recv_parts;
 		_grid_flush = c.flush_grid;
 		_grid_tiles = std::min( v.ntiles * v.rparts, ( 16 * 1024 / c.recv_block ) * cus );
 		_lc = c;
 		// statistics slots: one per workgroup of the widest step kernel (receive: per work item)
-		// (flush: two sets, the flushes of consecutive steps may overlap; tiled flush: per tile)
-		std::size_t const slots = std::size_t( std::max( { _grid_update, 2 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts } ) ) * NSTAT;
+		// (flush: two sets, the flushes of consecutive steps may overlap; eager passes: four, per
+		// pass and step parity; tiled flush: per tile)
+		std::size_t const slots = std::size_t( std::max( { _grid_update, 4 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts } ) ) * NSTAT;
 		if( slots > _bstats.size() )
 		{
 			std::vector<unsigned long long> keep( NSTAT, 0ull );
@@ -751,36 +817,66 @@ private:
 	}
 
 	// Enqueues the kernels of step `t` (t mod 4 selects the clock slot). Plastic models: the
-	// flush of step t goes to the side lane after step t's update and overlaps step t + 1;
-	// step t's delivery first waits for the flush of step t - 2 (`head`: the first two steps
-	// of a captured graph, whose earlier flushes finished before the graph started).
-	void launch_step( std::int64_t t, bool head = false )
+	// side work of step t (lazy flush, or eager pre and post passes) goes to the side lane
+	// after step t's update and overlaps step t + 1; step t's delivery first waits for the
+	// side work of step t - 2. `fresh`: steps of the graph being captured before step t (2:
+	// two or more, or not capturing) -- side work recorded before the capture finished before
+	// the graph started, and is not waited for.
+	void launch_step( std::int64_t t, int fresh = 2 )
 	{
 		dview const & v = _vs[t & ( CLOCK_SLOTS - 1 )];
 		if( fixed() )
-			launch_step_t<true>( v, t, head );
+			launch_step_t<true>( v, t, fresh );
 		else
-			launch_step_t<false>( v, t, head );
+			launch_step_t<false>( v, t, fresh );
 	}
 
 	template <bool F>
-	void launch_step_t( dview const & v, std::int64_t t, bool head )
+	void launch_step_t( dview const & v, std::int64_t t, int fresh )
 	{
 		if constexpr( plastic )
-			if( !head ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
+			if( fresh >= 2 )
+			{
+				SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
+				if constexpr( traced ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done2[t & 1], 0 ) );
+			}
 		launch_receive<F>( v );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
-		if constexpr( plastic )
+		if constexpr( traced )
 		{
+			// pre passes in order on s[0], post passes in order on s[1]; a step's two passes
+			// write disjoint synapses, but the post pass of step t may write synapses of rows
+			// that the pre pass of step t - 1 writes, so it waits for that one
+			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
+			SPICE_HIP( hipStreamWaitEvent( _lane.s[0], _lane.up, 0 ) );
+			SPICE_LAUNCH( k_stdp<Model, F, false><<<_grid_flush, 256, 0, _lane.s[0]>>>( v ) );
+			SPICE_HIP( hipStreamWaitEvent( _lane.s[1], _lane.up, 0 ) );
+			if( fresh >= 1 ) SPICE_HIP( hipStreamWaitEvent( _lane.s[1], _lane.done[( t - 1 ) & 1], 0 ) );
+			SPICE_HIP( hipEventRecord( _lane.done[t & 1], _lane.s[0] ) );
+			SPICE_LAUNCH( k_stdp<Model, F, true><<<_grid_flush, 256, 0, _lane.s[1]>>>( v ) );
+			SPICE_HIP( hipEventRecord( _lane.done2[t & 1], _lane.s[1] ) );
+		}
+		else if constexpr( plastic )
+		{
+			// the flushes of consecutive steps touch different rows and alternate between two
+			// side streams
 			hipStream_t const ls = _lane.s[t & 1];
 			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
 			SPICE_HIP( hipStreamWaitEvent( ls, _lane.up, 0 ) );
-			if( _p.flags & SPICE_FLAG_FLUSH_TILED )
-				SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, ls>>>( v ) );
-			else
-				SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, ls>>>( v ) );
+			launch_flush<F>( v, ls );
 			SPICE_HIP( hipEventRecord( _lane.done[t & 1], ls ) );
 		}
+	}
+
+	// SPICE_FLAG_FLUSH_TILED selects the target-tiled lazy flush
+	template <bool F>
+	void launch_flush( dview const & v, hipStream_t st )
+	{
+		static_assert( !traced, "neuron-trace synapses use the eager passes" );
+		if( _p.flags & SPICE_FLAG_FLUSH_TILED )
+			SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, st>>>( v ) );
+		else
+			SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, st>>>( v ) );
 	}
 
 	template <bool F>
@@ -802,13 +898,13 @@ private:
 		SPICE_HIP( hipEventRecord( e[1], _s ) );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[2], _s ) );
-		if constexpr( plastic )
+		if constexpr( traced )
 		{
-			if( _p.flags & SPICE_FLAG_FLUSH_TILED )
-				SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, _s>>>( v ) );
-			else
-				SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, _s>>>( v ) );
+			SPICE_LAUNCH( k_stdp<Model, F, false><<<_grid_flush, 256, 0, _s>>>( v ) );
+			SPICE_LAUNCH( k_stdp<Model, F, true><<<_grid_flush, 256, 0, _s>>>( v ) );
 		}
+		else if constexpr( plastic )
+			launch_flush<F>( v, _s );
 		SPICE_HIP( hipEventRecord( e[3], _s ) );
 	}
 
@@ -818,7 +914,7 @@ private:
 		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
 		try
 		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step( i, i < 2 ); // clock slots 0, 1, 2, 3, 0, ...
+			for( int i = 0; i < graph_steps; i++ ) launch_step( i, i ); // clock slots 0, 1, 2, 3, 0, ...
 			_lane.join_into( _s );                                          // the graph ends with its flushes
 		}
 		catch( ... )
@@ -837,7 +933,19 @@ private:
 	void sync_synapses()
 	{
 		synchronize();
-		if constexpr( plastic )
+		if constexpr( traced )
+		{
+			// the last step's events of the rows arriving next (once: the delivery then skips them)
+			if( _step > 0 && _caught_host != _step - 1 )
+			{
+				SPICE_LAUNCH( k_sync_eager<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, _step - 1 ) );
+				std::int64_t const now = _step - 1;
+				SPICE_HIP( hipMemcpyAsync( _caught.data(), &now, 8, hipMemcpyHostToDevice, _s ) );
+				SPICE_HIP( hipStreamSynchronize( _s ) ); // `now` is a host temporary
+				_caught_host = now;
+			}
+		}
+		else if constexpr( plastic )
 		{
 			if( _step > 0 )
 			{

@@ -65,7 +65,9 @@ inline std::uint64_t estimate_job_bytes( int model, std::int64_t n, std::int64_t
 		if constexpr( is_plastic_v<M> )
 		{
 			b += 8 * HWORDS * N + 36 * N;
-			if constexpr( has_neuron_traces_v<Ss> ) b += 8 * N + 8 * HISTORY * N;
+			// eager: traces, flags and target lists by parity, by-target index (at most every
+			// edge) with its offsets and build temporaries
+			if constexpr( has_neuron_traces_v<Ss> ) b += 8 * E + 48 * N + 64;
 		}
 		b += 8 * NSTAT * ( N / 256 + 8 * cus + ntiles + 1 );             // statistics slots
 		b += 4 * spike_log_capacity( n, chunk, dt ) + 8 * std::uint64_t( chunk ) + 64;

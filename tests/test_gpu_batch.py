@@ -49,9 +49,10 @@ def test_batch_equals_networks_alone(gpu, model):
 
 @pytest.mark.parametrize("model", ["brunel_3pop_stdp", "brunel_with_plasticity"])
 def test_plastic_batch_equals_networks_alone(gpu, model):
-    """Plastic nets in one batch (batched receive, update and lazy-STDP flush launches): spikes,
-    neuron state, synaptic weights and counters equal each net run alone, over enough steps
-    for every row to be flushed several times (window 255 steps)."""
+    """Plastic nets in one batch (batched receive, update and plasticity launches: eager passes
+    for brunel_3pop_stdp, lazy flushes for brunel_with_plasticity): spikes, neuron state,
+    synaptic weights and counters equal each net run alone, over enough steps for every lazy
+    row to be flushed several times (window 255 steps)."""
     rng = np.random.default_rng(11)
     nets = []
     for k in range(5):

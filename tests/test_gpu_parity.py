@@ -129,17 +129,28 @@ def test_lazy_stdp_generic_contract_matches_dense(gpu):
 
 
 @pytest.mark.parametrize("delay", [1, 15, 255])
-def test_lazy_stdp_neuron_traces_bit_exact(gpu, delay):
-    """Trace-synapse plasticity (brunel_3pop_stdp): event-only lazy replay with neuron-local
-    traces equals the checker's dense per-step plasticity bit for bit (fixed-point mode):
-    spikes every step, neuron state and every weight, including flushes of rows older than the
-    256-step history window (900 steps) and the largest supported delay (255)."""
+def test_eager_stdp_neuron_traces_bit_exact(gpu, delay):
+    """Trace-synapse plasticity (brunel_3pop_stdp), applied eagerly -- pre pass over arriving
+    rows, post pass over spiking targets through the by-target index, events of rows arriving
+    next left to that delivery -- equals the checker's dense per-step plasticity bit for bit
+    (fixed-point mode): spikes every step, neuron state, and every weight mid-run (an export
+    applies the deferred events once and the run goes on) and at the end; delays 1, 15 and the
+    largest supported (255)."""
     model = "brunel_3pop_stdp"
     lay = three_pop_layout(1500)
-    h, d, _ = _run_pair(model, lay, delay, 900, True, check_every=100)
+    h = O.CpuSnn(model, lay, 1e-4, delay, seed=5, fixed_point=True)
+    d = Snn(model, Layout(*lay), 1e-4, delay, seed=5, fixed_point=True)
+    for i in range(900):
+        sh = np.sort(h.step(True))
+        sd = np.sort(d.step(True))
+        np.testing.assert_array_equal(sd, sh, err_msg=f"step {i}: spike sets differ")
+        if i % 100 == 0:
+            assert_state_equal(d.neurons(), h.neurons(), f"step {i}")
+        if i in (299, 300, 601):  # twice in a row: the deferred events are applied only once
+            assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), f"weights after step {i}")
     assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), "weights")
     st = d.stats()
-    assert st["plastic_events"] > 0 and st["flush_rows"] > 0
+    assert st["plastic_events"] > 0 and st["flush_rows"] > 0 and st["flush_synapses"] > 0
 
 
 def test_run_graph_equals_step(gpu):
@@ -219,9 +230,10 @@ def test_autotune_keeps_the_network_in_place(gpu):
 
 @pytest.mark.parametrize("delay", [1, 15])
 def test_trace_stdp_graph_replay_bit_exact(gpu, delay):
-    """HIP-graph replay of plastic steps (the flush of each step overlapping the next steps on
-    two side streams) equals the dense checker bit for bit over 1,500 steps (several flush
-    windows), with autotuned-style launch shapes (split delivery lists, small flush items)."""
+    """HIP-graph replay of plastic steps (the plasticity passes of each step overlapping the
+    next step on a side stream) equals the dense checker bit for bit over 1,500 steps, weights
+    exported between the graph runs, with autotuned-style launch shapes (split delivery lists,
+    small flush items)."""
     model = "brunel_3pop_stdp"
     lay = three_pop_layout(2000)
     h = O.CpuSnn(model, lay, 1e-4, delay, seed=13, fixed_point=True)
@@ -231,6 +243,6 @@ def test_trace_stdp_graph_replay_bit_exact(gpu, delay):
         h.run(500)
         d.run(500)  # 500 = 15 graphs of 32 steps + eager steps
         assert_state_equal(d.neurons(), h.neurons(), f"after {500 * (k + 1)} steps")
-    assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), "weights")
+        assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), f"weights after {500 * (k + 1)} steps")
     st = d.stats()
     assert st["plastic_events"] > 0 and st["flush_rows"] > 0 and st["flush_synapses"] > 0
