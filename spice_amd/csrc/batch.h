@@ -30,6 +30,7 @@ class hip_batch final : public batch_iface
 {
 	static constexpr int graph_steps = 32;
 	static constexpr bool plastic = is_plastic_v<Model>;
+	static constexpr bool traced = plastic && has_neuron_traces_v<typename Model::synapse>;
 	// delivery workgroup size: most tiles of a sweep belong to small nets and carry few events,
 	// so more, smaller workgroups shorten the launch's critical path
 	static constexpr int recv_block = SPICE_BATCH_RECV_BLOCK;
@@ -42,7 +43,7 @@ public:
 	    , _owned( std::move( owned ) )
 	    , _s( stream )
 	    , _own_stream( own_stream )
-	    , _lane( plastic )
+	    , _lane( plastic && !traced )
 	{
 		if( _nets.empty() ) throw std::invalid_argument( "empty batch" );
 		_fixed = _nets[0]->fixed();
@@ -164,7 +165,7 @@ private:
 	std::vector<std::unique_ptr<hip_net<Model>>> _owned;
 	hipStream_t _s = nullptr;
 	bool _own_stream = false;
-	side_lane _lane; // plastic models: the flush of step T overlaps step T + 1
+	side_lane _lane; // lazy plasticity: the flush of step T overlaps step T + 1
 	hipGraphExec_t _exec = nullptr;
 	bool _fixed = false;
 	std::int64_t _step = 0;
@@ -193,30 +194,16 @@ private:
 	template <bool F>
 	void launch_step_t( std::int64_t t, int fresh )
 	{
-		constexpr bool traced = has_neuron_traces_v<typename Model::synapse>;
+		constexpr bool lazy = plastic && !traced;
 		dview const * v = _views[t & ( CLOCK_SLOTS - 1 )].data();
-		if constexpr( plastic )
-			if( fresh >= 2 )
-			{
-				SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
-				if constexpr( traced ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done2[t & 1], 0 ) );
-			}
-		SPICE_LAUNCH( k_batch_receive<Model, F, recv_block><<<_grid_tiles, recv_block, 0, _s>>>( v, _tile_net.data(), _tile_local.data(), _ntiles ) );
+		if constexpr( lazy )
+			if( fresh >= 2 ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
+		// eager plasticity: the nets' step t - 1 passes run in the delivery kernel's extra workgroups
+		unsigned const grid = unsigned( _grid_tiles + ( traced ? _nflush : 0 ) );
+		SPICE_LAUNCH( k_batch_receive<Model, F, recv_block><<<grid, recv_block, 0, _s>>>( v, _tile_net.data(), _tile_local.data(), _ntiles, _grid_tiles,
+		                                                                                    _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
 		SPICE_LAUNCH( k_batch_update<Model, F><<<_nblocks, 256, 0, _s>>>( v, _block_net.data(), _block_local.data() ) );
-		if constexpr( traced )
-		{
-			// same lane rule as hip_net::launch_step_t: pre passes on s[0], post passes on s[1]
-			// after the previous step's pre pass
-			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
-			SPICE_HIP( hipStreamWaitEvent( _lane.s[0], _lane.up, 0 ) );
-			SPICE_LAUNCH( k_batch_stdp<Model, F, false><<<_nflush, 256, 0, _lane.s[0]>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
-			SPICE_HIP( hipStreamWaitEvent( _lane.s[1], _lane.up, 0 ) );
-			if( fresh >= 1 ) SPICE_HIP( hipStreamWaitEvent( _lane.s[1], _lane.done[( t - 1 ) & 1], 0 ) );
-			SPICE_HIP( hipEventRecord( _lane.done[t & 1], _lane.s[0] ) );
-			SPICE_LAUNCH( k_batch_stdp<Model, F, true><<<_nflush, 256, 0, _lane.s[1]>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
-			SPICE_HIP( hipEventRecord( _lane.done2[t & 1], _lane.s[1] ) );
-		}
-		else if constexpr( plastic )
+		if constexpr( lazy )
 		{
 			hipStream_t const ls = _lane.s[t & 1];
 			SPICE_HIP( hipEventRecord( _lane.up, _s ) );

@@ -27,17 +27,14 @@ inline void check( hipError_t e, char const * what )
 		::spice::hipx::check( hipGetLastError(), "kernel launch" );   \
 	} while( 0 )
 
-// Side streams and events for plasticity work that overlaps the following step. Lazy flushes:
-// the flush of step T runs on stream s[T & 1], so the flushes of two consecutive steps may run
-// at the same time (their rows are disjoint). Eager passes: pre passes run in order on s[0],
-// post passes in order on s[1], and the post pass of step T also waits for the pre pass of
-// step T - 1. Inactive if `on` is false.
+// Side streams and events for work that overlaps the following steps: the lazy-plasticity
+// flush of step T runs on stream s[T & 1] beside the delivery and update of steps T + 1 (and
+// T + 2), so the flushes of two consecutive steps may run at the same time (their rows are
+// disjoint). Inactive if `on` is false.
 struct side_lane
 {
 	hipStream_t s[2] = { nullptr, nullptr };
-	// done[p] / done2[p]: completion of the latest side work recorded for step parity p on
-	// s[0] / s[1]
-	hipEvent_t up = nullptr, join = nullptr, done[2] = { nullptr, nullptr }, done2[2] = { nullptr, nullptr };
+	hipEvent_t up = nullptr, join = nullptr, done[2] = { nullptr, nullptr };
 
 	explicit side_lane( bool on )
 	{
@@ -45,7 +42,7 @@ struct side_lane
 		try
 		{
 			for( hipStream_t * x : { &s[0], &s[1] } ) check( hipStreamCreateWithFlags( x, hipStreamNonBlocking ), "hipStreamCreateWithFlags" );
-			for( hipEvent_t * e : { &up, &join, &done[0], &done[1], &done2[0], &done2[1] } )
+			for( hipEvent_t * e : { &up, &join, &done[0], &done[1] } )
 				check( hipEventCreateWithFlags( e, hipEventDisableTiming ), "hipEventCreateWithFlags" );
 		}
 		catch( ... )
@@ -87,7 +84,7 @@ private:
 	{
 		for( hipStream_t x : s )
 			if( x ) (void)hipStreamSynchronize( x );
-		for( hipEvent_t * e : { &up, &join, &done[0], &done[1], &done2[0], &done2[1] } )
+		for( hipEvent_t * e : { &up, &join, &done[0], &done[1] } )
 			if( *e ) (void)hipEventDestroy( *e );
 		for( hipStream_t & x : s )
 		{

@@ -304,9 +304,9 @@ __device__ inline void replay(
 //     index) whose source's spike arrives neither at T nor at T + 1;
 //   * the delivery of step T + 1 (deferred): rows whose spike arrives at T + 1, before the
 //     delivery reads them.
-// The two passes run as one kernel on a side stream while step T + 1 delivers and updates:
-// they touch no row that delivery reads, they read only step T's traces and flags (arrays by
-// step parity), and consecutive steps' passes run in order on one stream.
+// The two passes of step T run inside the delivery kernel of step T + 1, beside its delivery:
+// they touch no row that delivery reads, and they read only step T's traces and flags (arrays
+// by step parity), which the update of step T + 1 writes only after that kernel.
 
 // flag bits of fl8
 constexpr std::uint8_t F8_SPIKED = 1, F8_ARRIVED = 2, F8_ARRIVES_NEXT = 4;
@@ -683,21 +683,54 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	block_add_stats<2>( v, work, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
 }
 
-template <typename Model, bool FIXED, int BS>
-__global__ __launch_bounds__( BS ) void k_receive_tiled( dview v )
+template <typename Model, bool FIXED>
+__device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t blk, std::int64_t nblk, std::int64_t slot0 );
+
+// Eager plasticity (neuron-trace synapses) runs inside the delivery kernel: the pre and post
+// passes of step T - 1 touch no row that step T delivers, so workgroups beyond the delivery
+// ones apply them while delivery runs, and the next kernel in stream order (the update of
+// step T, then the delivery of step T + 1 with the passes of step T) sees them done. Skipped
+// when a synchronization has applied them already (v.caught = T - 1).
+template <typename Model, bool FIXED>
+__device__ inline void stdp_prev_block( dview const & v, std::int64_t blk, std::int64_t nblk )
 {
-	std::int32_t const nwork = v.ntiles * v.rparts;
-	for( std::int32_t w = blockIdx.x; w < nwork; w += gridDim.x ) receive_tile<Model, FIXED, BS>( v, w );
+	std::int64_t const T = v.ctl->step;
+	if( T >= 1 && *v.caught != T - 1 ) stdp_block<Model, FIXED>( v, T - 1, blk, nblk, std::int64_t( v.ntiles ) * v.rparts );
+}
+
+// Workgroups [0, grecv) deliver (work items round robin); eager plasticity models give the
+// workgroups [grecv, gridDim.x) to step T - 1's plasticity passes.
+template <typename Model, bool FIXED, int BS>
+__global__ __launch_bounds__( BS ) void k_receive_tiled( dview v, std::int32_t grecv )
+{
+	if( std::int32_t( blockIdx.x ) < grecv )
+	{
+		std::int32_t const nwork = v.ntiles * v.rparts;
+		for( std::int32_t w = blockIdx.x; w < nwork; w += grecv ) receive_tile<Model, FIXED, BS>( v, w );
+	}
+	else if constexpr( is_plastic_v<Model> && has_neuron_traces_v<typename Model::synapse> )
+		stdp_prev_block<Model, FIXED>( v, blockIdx.x - grecv, gridDim.x - grecv );
 }
 
 // Batched delivery: global work item gt belongs to net tile_net[gt] (local work item
-// tile_local[gt] = tile * rparts + part); the net's view is read from device memory.
+// tile_local[gt] = tile * rparts + part); the net's view is read from device memory. Eager
+// plasticity: workgroup grecv + b runs plasticity block block_local[b] of the block_count[b]
+// blocks of net block_net[b].
 template <typename Model, bool FIXED, int BS>
-__global__ __launch_bounds__( BS ) void k_batch_receive(
-    dview const * views, std::int32_t const * tile_net, std::int32_t const * tile_local, std::int32_t ntiles )
+__global__ __launch_bounds__( BS ) void k_batch_receive( dview const * views, std::int32_t const * tile_net, std::int32_t const * tile_local,
+                                                         std::int32_t ntiles, std::int32_t grecv, std::int32_t const * block_net,
+                                                         std::int32_t const * block_local, std::int32_t const * block_count )
 {
-	for( std::int32_t gt = blockIdx.x; gt < ntiles; gt += gridDim.x )
-		receive_tile<Model, FIXED, BS>( views[tile_net[gt]], tile_local[gt] );
+	if( std::int32_t( blockIdx.x ) < grecv )
+	{
+		for( std::int32_t gt = blockIdx.x; gt < ntiles; gt += grecv )
+			receive_tile<Model, FIXED, BS>( views[tile_net[gt]], tile_local[gt] );
+	}
+	else if constexpr( is_plastic_v<Model> && has_neuron_traces_v<typename Model::synapse> )
+	{
+		std::int32_t const b = blockIdx.x - grecv;
+		stdp_prev_block<Model, FIXED>( views[block_net[b]], block_local[b], block_count[b] );
+	}
 }
 
 
@@ -1007,18 +1040,19 @@ __global__ __launch_bounds__( 256 ) void k_batch_update( dview const * views, st
 //              synapse whose source's spike arrives at T or T + 1 is left to the pre pass or
 //              to the next delivery, the others get a post event with the source's pre trace.
 // Both kinds write disjoint synapses (a synapse is in a pre row iff its source arrived at T).
+// Statistics go to slots slot0 + blk.
 #ifndef SPICE_POST_UNROLL
 #define SPICE_POST_UNROLL 4
 #endif
-template <typename Model, bool FIXED, bool POST>
-__device__ inline void stdp_block( dview const & v, std::int64_t blk, std::int64_t nblk )
+template <typename Model, bool FIXED>
+__device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t blk, std::int64_t nblk, std::int64_t slot0 )
 {
 	using S = typename Model::synapse;
 	constexpr int U = 4;                  // pre pass: synapses per lane per iteration in flight
 	constexpr int UP = SPICE_POST_UNROLL; // post pass: by-target entries per lane per iteration
-	std::int64_t const T = v.ctl->step;
 	std::int64_t const cur = ( T & 1 ) * std::int64_t( v.N ); // step T's lists, traces and flags
-	std::int64_t const nitems = POST ? std::int64_t( v.po_count[T & 3] ) * v.pparts : std::int64_t( v.fl_count[T & 3] ) * v.fparts;
+	std::int64_t const npre = std::int64_t( v.fl_count[T & 3] ) * v.fparts;
+	std::int64_t const nitems = npre + std::int64_t( v.po_count[T & 3] ) * v.pparts;
 	snn_info const info = info_of( v );
 	int const lane = lane_id();
 	std::int64_t const wave0 = uniform64( ( blk * blockDim.x + threadIdx.x ) >> 6 );
@@ -1026,7 +1060,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t blk, std::int64
 	unsigned long long touched = 0, words = 0, rows = 0;
 	for( std::int64_t it = wave0; it < nitems; it += nwaves )
 	{
-		if constexpr( !POST )
+		if( it < npre )
 		{
 			std::int64_t const f = it / v.fparts;
 			std::int64_t const p = it - f * v.fparts;
@@ -1084,8 +1118,9 @@ __device__ inline void stdp_block( dview const & v, std::int64_t blk, std::int64
 		}
 		else
 		{
-			std::int64_t const f = it / v.pparts;
-			std::int64_t const p = it - f * v.pparts;
+			std::int64_t const q = it - npre;
+			std::int64_t const f = q / v.pparts;
+			std::int64_t const p = q - f * v.pparts;
 			std::int32_t const d = uniform( v.po_id[cur + f] );
 			std::int64_t const lo = v.tp_ptr[d], hi = v.tp_ptr[d + 1];
 			std::int64_t const chunk = ( ( hi - lo + v.pparts - 1 ) / v.pparts + 63 ) & ~std::int64_t( 63 );
@@ -1138,26 +1173,15 @@ __device__ inline void stdp_block( dview const & v, std::int64_t blk, std::int64
 			}
 		}
 	}
-	// statistics slots: pre and post passes of the same or adjacent steps may run at once, so
-	// one set per (pass, step parity)
-	block_add_stats<3>( v, blk + ( ( POST ? 2 : 0 ) + ( T & 1 ) ) * nblk, { touched, words, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	block_add_stats<3>( v, slot0 + blk, { touched, words, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
-// Eager plasticity of step T: the pre pass (POST = false) or the post pass (POST = true).
-template <typename Model, bool FIXED, bool POST>
-__global__ __launch_bounds__( 256 ) void k_stdp( dview v )
+// Eager plasticity of step `T` on its own (a synchronization before a state export applies
+// the passes of the last completed step that the next delivery kernel would otherwise run).
+template <typename Model, bool FIXED>
+__global__ __launch_bounds__( 256 ) void k_stdp( dview v, std::int64_t T, std::int64_t slot0 )
 {
-	stdp_block<Model, FIXED, POST>( v, blockIdx.x, gridDim.x );
-}
-
-// Batched eager pass: workgroup b is block block_local[b] of the block_count[b] blocks of net
-// block_net[b].
-template <typename Model, bool FIXED, bool POST>
-__global__ __launch_bounds__( 256 ) void k_batch_stdp(
-    dview const * views, std::int32_t const * block_net, std::int32_t const * block_local, std::int32_t const * block_count )
-{
-	std::int32_t const b = blockIdx.x;
-	stdp_block<Model, FIXED, POST>( views[block_net[b]], block_local[b], block_count[b] );
+	stdp_block<Model, FIXED>( v, T, blockIdx.x, gridDim.x, slot0 );
 }
 
 // Row-major flush of the rows whose history window is full (lazy plasticity of the

@@ -109,7 +109,7 @@ public:
 	    : _p( p )
 	    , _g( std::move( g ) )
 	    , _s( stream )
-	    , _lane( plastic )
+	    , _lane( plastic && !traced )
 	{
 		if( !( p.dt > 0.0f ) ) throw std::invalid_argument( "dt must be positive" );
 		if( p.delay < 1 ) throw std::invalid_argument( "delay must be >= 1" );
@@ -518,7 +518,7 @@ private:
 	net_params _p;
 	csr_dev _g;
 	hipStream_t _s = nullptr;
-	side_lane _lane; // plastic models: the flush of step T overlaps step T + 1
+	side_lane _lane; // lazy plasticity: the flush of step T overlaps step T + 1
 	hipGraphExec_t _exec = nullptr;
 
 	dbuf<char> _na[MAXA], _acc[MAXA], _sa[MAXA];
@@ -545,6 +545,7 @@ private:
 	dview _vs[CLOCK_SLOTS]{};
 
 	int _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
+	int _grid_plast = 0; // eager plasticity workgroups of the delivery kernel
 	launch_cfg _lc{};
 	dbuf<std::int32_t> _pivots;
 	std::int64_t _step = 0;
@@ -770,11 +771,13 @@ private:
 		v.rparts = c.recv_parts;
 		_grid_flush = c.flush_grid;
 		_grid_tiles = std::min( v.ntiles * v.rparts, ( 16 * 1024 / c.recv_block ) * cus );
+		_grid_plast = traced ? std::max( 1, _grid_flush * 256 / c.recv_block ) : 0; // the same waves as flush_grid
 		_lc = c;
-		// statistics slots: one per workgroup of the widest step kernel (receive: per work item)
-		// (flush: two sets, the flushes of consecutive steps may overlap; eager passes: four, per
-		// pass and step parity; tiled flush: per tile)
-		std::size_t const slots = std::size_t( std::max( { _grid_update, 4 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts } ) ) * NSTAT;
+		// statistics slots: one per workgroup of the widest step kernel (receive: per work item;
+		// eager plasticity: after those, one per plasticity workgroup or synchronization block)
+		// (flush: two sets, the flushes of consecutive steps may overlap; tiled flush: per tile)
+		std::size_t const slots =
+		    std::size_t( std::max( { _grid_update, 2 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts + std::max( _grid_plast, _grid_flush ) } ) ) * NSTAT;
 		if( slots > _bstats.size() )
 		{
 			std::vector<unsigned long long> keep( NSTAT, 0ull );
@@ -816,12 +819,12 @@ private:
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
-	// Enqueues the kernels of step `t` (t mod 4 selects the clock slot). Plastic models: the
-	// side work of step t (lazy flush, or eager pre and post passes) goes to the side lane
-	// after step t's update and overlaps step t + 1; step t's delivery first waits for the
-	// side work of step t - 2. `fresh`: steps of the graph being captured before step t (2:
-	// two or more, or not capturing) -- side work recorded before the capture finished before
-	// the graph started, and is not waited for.
+	// Enqueues the kernels of step `t` (t mod 4 selects the clock slot). Lazy plasticity: the
+	// flush of step t goes to the side lane after step t's update and overlaps step t + 1;
+	// step t's delivery first waits for the flush of step t - 2. `fresh`: steps of the graph
+	// being captured before step t (2: two or more, or not capturing) -- flushes recorded
+	// before the capture finished before the graph started, and are not waited for. Eager
+	// plasticity: step t - 1's passes run inside step t's delivery kernel.
 	void launch_step( std::int64_t t, int fresh = 2 )
 	{
 		dview const & v = _vs[t & ( CLOCK_SLOTS - 1 )];
@@ -834,29 +837,12 @@ private:
 	template <bool F>
 	void launch_step_t( dview const & v, std::int64_t t, int fresh )
 	{
-		if constexpr( plastic )
-			if( fresh >= 2 )
-			{
-				SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
-				if constexpr( traced ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done2[t & 1], 0 ) );
-			}
+		constexpr bool lazy = plastic && !traced;
+		if constexpr( lazy )
+			if( fresh >= 2 ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
 		launch_receive<F>( v );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
-		if constexpr( traced )
-		{
-			// pre passes in order on s[0], post passes in order on s[1]; a step's two passes
-			// write disjoint synapses, but the post pass of step t may write synapses of rows
-			// that the pre pass of step t - 1 writes, so it waits for that one
-			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
-			SPICE_HIP( hipStreamWaitEvent( _lane.s[0], _lane.up, 0 ) );
-			SPICE_LAUNCH( k_stdp<Model, F, false><<<_grid_flush, 256, 0, _lane.s[0]>>>( v ) );
-			SPICE_HIP( hipStreamWaitEvent( _lane.s[1], _lane.up, 0 ) );
-			if( fresh >= 1 ) SPICE_HIP( hipStreamWaitEvent( _lane.s[1], _lane.done[( t - 1 ) & 1], 0 ) );
-			SPICE_HIP( hipEventRecord( _lane.done[t & 1], _lane.s[0] ) );
-			SPICE_LAUNCH( k_stdp<Model, F, true><<<_grid_flush, 256, 0, _lane.s[1]>>>( v ) );
-			SPICE_HIP( hipEventRecord( _lane.done2[t & 1], _lane.s[1] ) );
-		}
-		else if constexpr( plastic )
+		if constexpr( lazy )
 		{
 			// the flushes of consecutive steps touch different rows and alternate between two
 			// side streams
@@ -879,15 +865,17 @@ private:
 			SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, st>>>( v ) );
 	}
 
+	// delivery workgroups, plus (eager plasticity) the previous step's plasticity workgroups
 	template <bool F>
 	void launch_receive( dview const & v )
 	{
+		unsigned const grid = unsigned( _grid_tiles + _grid_plast );
 		if( _lc.recv_block == 256 )
-			SPICE_LAUNCH( k_receive_tiled<Model, F, 256><<<_grid_tiles, 256, 0, _s>>>( v ) );
+			SPICE_LAUNCH( k_receive_tiled<Model, F, 256><<<grid, 256, 0, _s>>>( v, _grid_tiles ) );
 		else if( _lc.recv_block == 512 )
-			SPICE_LAUNCH( k_receive_tiled<Model, F, 512><<<_grid_tiles, 512, 0, _s>>>( v ) );
+			SPICE_LAUNCH( k_receive_tiled<Model, F, 512><<<grid, 512, 0, _s>>>( v, _grid_tiles ) );
 		else
-			SPICE_LAUNCH( k_receive_tiled<Model, F, 1024><<<_grid_tiles, 1024, 0, _s>>>( v ) );
+			SPICE_LAUNCH( k_receive_tiled<Model, F, 1024><<<grid, 1024, 0, _s>>>( v, _grid_tiles ) );
 	}
 
 	template <bool F>
@@ -898,13 +886,7 @@ private:
 		SPICE_HIP( hipEventRecord( e[1], _s ) );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[2], _s ) );
-		if constexpr( traced )
-		{
-			SPICE_LAUNCH( k_stdp<Model, F, false><<<_grid_flush, 256, 0, _s>>>( v ) );
-			SPICE_LAUNCH( k_stdp<Model, F, true><<<_grid_flush, 256, 0, _s>>>( v ) );
-		}
-		else if constexpr( plastic )
-			launch_flush<F>( v, _s );
+		if constexpr( plastic && !traced ) launch_flush<F>( v, _s );
 		SPICE_HIP( hipEventRecord( e[3], _s ) );
 	}
 
@@ -935,9 +917,16 @@ private:
 		synchronize();
 		if constexpr( traced )
 		{
-			// the last step's events of the rows arriving next (once: the delivery then skips them)
+			// the last step's plasticity passes and its events of the rows arriving next (once:
+			// the next delivery kernel then skips both)
 			if( _step > 0 && _caught_host != _step - 1 )
 			{
+				dview const & v = _vs[( _step - 1 ) & ( CLOCK_SLOTS - 1 )];
+				std::int64_t const slot0 = std::int64_t( v.ntiles ) * v.rparts;
+				if( fixed() )
+					SPICE_LAUNCH( k_stdp<Model, true><<<_grid_flush, 256, 0, _s>>>( v, _step - 1, slot0 ) );
+				else
+					SPICE_LAUNCH( k_stdp<Model, false><<<_grid_flush, 256, 0, _s>>>( v, _step - 1, slot0 ) );
 				SPICE_LAUNCH( k_sync_eager<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, _step - 1 ) );
 				std::int64_t const now = _step - 1;
 				SPICE_HIP( hipMemcpyAsync( _caught.data(), &now, 8, hipMemcpyHostToDevice, _s ) );
