@@ -169,7 +169,8 @@ public:
 	}
 
 	// Runs n steps eagerly with HIP events around every kernel on the net's stream and adds
-	// the summed kernel times (ms) to kernel_ms[0..2] = receive, update, flush.
+	// the summed kernel times (ms) to kernel_ms[0..2] = receive, update, flush (lazy
+	// plasticity; eager plasticity runs inside the receive kernel).
 	void run_profiled( std::int64_t n, double * kernel_ms ) override
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
@@ -198,7 +199,7 @@ public:
 					hipEvent_t * e = &ev[i * ev_per_step];
 					for( int q = 0; q < 3; q++ )
 					{
-						if( q == 2 && !plastic ) continue;
+						if( q == 2 && !( plastic && !traced ) ) continue;
 						float ms = 0;
 						SPICE_HIP( hipEventElapsedTime( &ms, e[q], e[q + 1] ) );
 						kernel_ms[q] += ms;
