@@ -82,10 +82,10 @@ struct synapse : util::type_list<Ts...>
 //   on_event(syn, pre = arrived_src(t), post = spiked_dst(t), x_src(t), y_dst(t), info, backend)
 // The synapse state holds only what the rule needs per synapse (typically the weight).
 // on_event must leave the synapse unchanged when pre and post are both false; that is what
-// lets the GPU apply it only at events (lazily) with results identical to the dense
-// per-step definition. It reads x only when post is set and y only when pre is set (the
-// lazy replay passes 0 for a trace the event does not read, so it is never loaded). The
-// synapse::update of the generic contract is not used.
+// lets the GPU apply it only at events (eagerly, in or right after the event's step) with
+// results identical to the dense per-step definition. It reads x only when post is set and y
+// only when pre is set (the GPU passes 0 for a trace the event does not read, so it is never
+// loaded). The synapse::update of the generic contract is not used.
 template <typename... Ts>
 struct trace_synapse : synapse<Ts...>
 {

@@ -109,9 +109,13 @@ typedef struct
 	uint64_t spikes;         /* spikes emitted */
 	uint64_t events;         /* synaptic events delivered (real edges of delivered spikes) */
 	uint64_t plastic_events; /* plastic synapses changed when brought up to date on delivery */
-	uint64_t flush_rows;     /* rows flushed because their post-spike history window filled */
-	uint64_t flush_synapses; /* plastic synapses changed by those flushes */
-	uint64_t flush_slots;    /* synapse slots (row entries) scanned by those flushes */
+	/* Plasticity passes. Lazy (per-synapse contract): rows flushed (spike arrived or history
+	 * window full), plastic synapses changed, row slots scanned. Eager (neuron-trace
+	 * synapses): pre-pass rows + post-pass targets, synapses changed, 4-byte index words read
+	 * (one per pre-pass slot, two per post-pass entry). */
+	uint64_t flush_rows;
+	uint64_t flush_synapses;
+	uint64_t flush_slots;
 } spice_stats;
 
 /* Thread-local message of the last failed call on this thread ("" if none). */
@@ -217,8 +221,10 @@ int spice_hip_run( spice_net * net, int64_t nsteps );
 
 /* Enqueues nsteps steps without graph capture, with HIP events recorded around every kernel
  * on the net's stream, waits, and adds the summed per-kernel device times in milliseconds to
- * kernel_ms[0] (spike delivery), kernel_ms[1] (neuron update) and kernel_ms[2] (plasticity
- * flush; plastic models only). Same results as spice_hip_run. Used for roofline figures. */
+ * kernel_ms[0] (spike delivery; for neuron-trace plasticity including the previous step's
+ * plasticity passes, which run in the same kernel), kernel_ms[1] (neuron update) and
+ * kernel_ms[2] (lazy-plasticity flush kernel; models of the per-synapse contract only). Same
+ * results as spice_hip_run. Used for roofline figures. */
 int spice_hip_run_profiled( spice_net * net, int64_t nsteps, double * kernel_ms );
 
 /* Waits for all work of the net. */
@@ -242,8 +248,9 @@ int spice_batch_net( spice_batch * batch, int32_t i, spice_net ** out_net );
 
 /* ---- launch shapes and the autotuner --------------------------------------------------- */
 /* No counterpart in the reference (its launch shapes are compile-time constants). tile_w:
- * targets per spike-delivery tile (multiple of 64, 64..1024); flush_slots: target slots per
- * lazy-plasticity flush work item (multiple of 64); flush_grid: flush workgroups;
+ * targets per spike-delivery tile (multiple of 64, 64..1024); flush_slots: synapses per
+ * plasticity work item (a lazy flush's row slice, or an eager pass's row or target slice;
+ * multiple of 64); flush_grid: plasticity workgroups (of 256 threads' worth);
  * recv_block: threads per delivery workgroup. A zero field selects the default. Launch shapes
  * never change results. */
 typedef struct spice_launch
@@ -259,8 +266,9 @@ typedef struct spice_launch
 int spice_hip_get_launch( spice_net * net, spice_launch * out );
 int spice_hip_set_launch( spice_net * net, const spice_launch * in );
 /* Times candidate launch shapes on a copy of the network (warmup steps first, then `steps`
- * graph-replayed steps per candidate, wall time per step: the flush of a step overlaps the
- * next step, so shapes are judged together; tile width first, then the flush shape),
+ * graph-replayed steps per candidate, wall time per step: plasticity overlaps delivery, so
+ * shapes are judged together; tile width first, then delivery parts, then the plasticity
+ * shape),
  * applies the fastest to the network and returns it with its step time (us). The network
  * itself does not advance. Results are cached per (model, neurons, edges, delay, flags) for
  * the life of the process. */
