@@ -56,7 +56,8 @@ enum spice_flag
 	 * depend on atomic order and match the CPU checker's fixed-point mode bit for bit. */
 	SPICE_FLAG_FIXED_POINT = 1u << 0,
 	/* Launch-shape choice (same results): bring queued plastic rows up to date with the
-	 * target-tiled kernel instead of the row-major one. */
+	 * target-tiled kernel instead of the row-major one (lazy plasticity, i.e. models of the
+	 * per-synapse contract; no effect for neuron-trace synapses, whose plasticity is eager). */
 	SPICE_FLAG_FLUSH_TILED = 1u << 1
 };
 
