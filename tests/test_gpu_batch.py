@@ -79,6 +79,37 @@ def test_plastic_batch_equals_networks_alone(gpu, model):
     b.close()
 
 
+@pytest.mark.parametrize("model", ["brunel_3pop_stdp", "brunel_with_plasticity"])
+def test_plastic_batch_export_mid_run_equals_alone(gpu, model):
+    """A batched net's weights exported mid-run (the export first brings that net's synapses up
+    to date: for eager plasticity it applies the last step's passes once, which the batch's next
+    delivery kernel then skips for that net only), then the batch goes on: every net equals the
+    same net run alone with the same exports, at both exports."""
+    rng = np.random.default_rng(12)
+    nets = []
+    for k, d in enumerate([1, 4, 15]):
+        n = int(rng.integers(500, 1500))
+        lay = three_pop_layout(n) if model == "brunel_3pop_stdp" else ([n // 2, n - n // 2], [(0, 1, 0.1), (1, 1, 0.1)])
+        nets.append((Layout(*lay), d, 700 + k, None))
+    b = Batch(model, nets, 1e-4, fixed_point=True)
+    alone = [Snn(model, lay, 1e-4, d, seed=s, fixed_point=True) for lay, d, s, _ in nets]
+    exported = [0, 2]  # nets exported after the first half; net 1 only at the end
+    for half in range(2):
+        b.run(333)
+        b.synchronize()
+        for k, a in enumerate(alone):
+            a.run(333)
+            if half == 0 and k not in exported:
+                continue
+            wa, wb = a.synapses(ell=False), b.net(k).synapses(ell=False)
+            for key in wa:
+                np.testing.assert_array_equal(wa[key], wb[key], err_msg=f"half {half} net {k} synapse attribute {key}")
+            assert_state_equal(b.net(k).neurons(), a.neurons(), f"half {half} net {k}")
+    for a in alone:
+        a.close()
+    b.close()
+
+
 def test_batch_net_parameters_match_checker(gpu):
     """A per-network Poisson rate (snn_info::param[0]) reaches the model identically on the GPU
     (inside a batch) and in the CPU checker."""
