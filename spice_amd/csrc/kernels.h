@@ -212,9 +212,10 @@ __device__ inline void store_tuple( typename Decl::tuple_t const & s, void * con
 	} );
 }
 
-// ---- lazy plasticity ---------------------------------------------------------------------
-// Spike history: bit (t mod HISTORY) of ring word ((t mod HISTORY) / 64) of neuron j is set
-// iff j spiked at step t, valid for the last HISTORY steps; last_spk[j] is j's latest spike.
+// ---- spike history and lazy plasticity --------------------------------------------------
+// Spike history (plastic models): bit (t mod HISTORY) of ring word ((t mod HISTORY) / 64) of
+// neuron j is set iff j spiked at step t, valid for the last HISTORY steps. Lazy plasticity
+// also keeps last_spk[j], j's latest spike.
 
 __device__ inline bool spiked_at( dview const & v, std::int32_t j, std::int32_t e )
 {
