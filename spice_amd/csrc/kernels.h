@@ -53,6 +53,29 @@ __device__ inline void st( void * base, std::int64_t i, T x )
 	static_cast<T *>( base )[i] = x;
 }
 
+// Streaming accesses (data touched once per step: synapse rows and scattered weights) with
+// the non-temporal hint, so they do not evict the small per-neuron arrays that the same
+// kernels gather from repeatedly. SPICE_NT bits: 1 post pass, 2 pre pass, 4 delivery.
+#ifndef SPICE_NT
+#define SPICE_NT 0
+#endif
+template <bool NT, typename T>
+__device__ inline T lds( T const * p )
+{
+	if constexpr( NT )
+		return __builtin_nontemporal_load( p );
+	else
+		return *p;
+}
+template <bool NT, typename T>
+__device__ inline void sts( T * p, T x )
+{
+	if constexpr( NT )
+		__builtin_nontemporal_store( x, p );
+	else
+		*p = x;
+}
+
 // ---- iterators handed to model functors ------------------------------------------------
 
 // State held in registers (neuron update, synapse replay).
@@ -195,20 +218,20 @@ __device__ inline void block_add_stats( dview const & v, std::int64_t slot, unsi
 }
 
 // ---- state load/store -------------------------------------------------------------------
-template <typename Decl>
+template <typename Decl, bool NT = false>
 __device__ inline void load_tuple( typename Decl::tuple_t & s, void * const * arr, std::int64_t i )
 {
 	util::for_n<Decl::size>( [&]( auto I ) {
 		using T = typename Decl::template type<I>;
-		std::get<I>( s ) = ld<T>( arr[I], i );
+		std::get<I>( s ) = lds<NT>( static_cast<T const *>( arr[I] ) + i );
 	} );
 }
-template <typename Decl>
+template <typename Decl, bool NT = false>
 __device__ inline void store_tuple( typename Decl::tuple_t const & s, void * const * arr, std::int64_t i )
 {
 	util::for_n<Decl::size>( [&]( auto I ) {
 		using T = typename Decl::template type<I>;
-		st<T>( arr[I], i, std::get<I>( s ) );
+		sts<NT>( static_cast<T *>( arr[I] ) + i, std::get<I>( s ) );
 	} );
 }
 
@@ -581,7 +604,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					std::int32_t const a = segment_of( seg_pre, cnt, f );
 					kk[u] = a;
 					e[u] = seg_beg[a] + ( f - seg_pre[a] );
-					j[u] = ok[u] ? v.col[e[u]] : 0;
+					j[u] = ok[u] ? lds<( SPICE_NT & 4 ) != 0>( v.col + e[u] ) : 0;
 				}
 				if constexpr( eager )
 				{
@@ -594,7 +617,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					float yv[U];
 #pragma unroll
 					for( int u = 0; u < U; u++ )
-						if( ok[u] ) load_tuple<S>( sy[u], v.sa, e[u] );
+						if( ok[u] ) load_tuple<S, ( SPICE_NT & 4 ) != 0>( sy[u], v.sa, e[u] );
 #pragma unroll
 					for( int u = 0; u < U; u++ )
 					{
@@ -617,7 +640,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 							auto sbak = make_backend<FIXED>( v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ),
 							                                 std::uint32_t( T - 1 ), util::TAG_SYNAPSE_UPDATE );
 							S::on_event( itr, pre, post, post ? seg_xf[kk[u]] : 0.0f, yv[u], info, sbak );
-							store_tuple<S>( sy[u], v.sa, e[u] );
+							store_tuple<S, ( SPICE_NT & 4 ) != 0>( sy[u], v.sa, e[u] );
 							pev++;
 						}
 						N::receive( s, dst, itr, info, bak );
@@ -1051,6 +1074,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 	using S = typename Model::synapse;
 	constexpr int U = 4;                  // pre pass: synapses per lane per iteration in flight
 	constexpr int UP = SPICE_POST_UNROLL; // post pass: by-target entries per lane per iteration
+	constexpr bool NTPOST = ( SPICE_NT & 1 ) != 0, NTPRE = ( SPICE_NT & 2 ) != 0;
 	std::int64_t const cur = ( T & 1 ) * std::int64_t( v.N ); // step T's lists, traces and flags
 	std::int64_t const npre = std::int64_t( v.fl_count[T & 3] ) * v.fparts;
 	std::int64_t const nitems = npre + std::int64_t( v.po_count[T & 3] ) * v.pparts;
@@ -1081,7 +1105,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 				{
 					std::int64_t const e = base + lane + 64 * u;
 					ok[u] = e < end;
-					j[u] = ok[u] ? v.col[e] : 0;
+					j[u] = ok[u] ? lds<NTPRE>( v.col + e ) : 0;
 				}
 				typename S::tuple_t sy[U];
 				std::uint8_t f8[U];
@@ -1092,7 +1116,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 					ok[u] = ok[u] && S::plastic( s, j[u], info );
 					if( ok[u] )
 					{
-						load_tuple<S>( sy[u], v.sa, base + lane + 64 * u );
+						load_tuple<S, NTPRE>( sy[u], v.sa, base + lane + 64 * u );
 						f8[u] = v.fl8[cur + j[u]];
 						yv[u] = v.ytr[cur + j[u]];
 					}
@@ -1107,7 +1131,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e ) };
 					bool const post = ( f8[u] & F8_SPIKED ) != 0;
 					S::on_event( itr, true, post, post ? xs : 0.0f, yv[u], info, bak );
-					store_tuple<S>( sy[u], v.sa, e );
+					store_tuple<S, NTPRE>( sy[u], v.sa, e );
 					touched++;
 				}
 			}
@@ -1154,7 +1178,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 				typename S::tuple_t sy[UP];
 #pragma unroll
 				for( int u = 0; u < UP; u++ )
-					if( ok[u] ) load_tuple<S>( sy[u], v.sa, e[u] );
+					if( ok[u] ) load_tuple<S, NTPOST>( sy[u], v.sa, e[u] );
 #pragma unroll
 				for( int u = 0; u < UP; u++ )
 				{
@@ -1163,7 +1187,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 					    v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
 					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
 					S::on_event( itr, false, true, xv[u], 0.0f, info, bak ); // post only: y is not read
-					store_tuple<S>( sy[u], v.sa, e[u] );
+					store_tuple<S, NTPOST>( sy[u], v.sa, e[u] );
 					touched++;
 				}
 			}
