@@ -80,8 +80,8 @@ def algorithmic_bytes(model, n, st, steps):
                event, has-plastic flag; 1 eager: the flag), for neuron-trace synapses (eager)
                the traces read + written (16) and the step's flags (1) per neuron;
                4 per emitted spike;
-      flush  : the plasticity pass (a side-stream kernel for lazy plasticity, counted with the
-               delivery kernel that runs it for eager plasticity): 4 per index word read (lazy flush: the target id
+      flush  : the plasticity work (counted with the delivery kernel that runs it): 4 per
+               index word read (lazy flush: the target id
                of every scanned slot of the flushed rows; eager: the target id of every plastic
                slot of the arriving rows and the (source, offset) pair of every by-target entry
                of the spiking targets) + per changed synapse its state read + write (8
@@ -101,9 +101,8 @@ def algorithmic_bytes(model, n, st, steps):
     recv = spk * 20 + ev * (8 if plastic else 4) + pev * (wb if plastic else 0)
     upd = n * (2 * s_n + 2 * acc_bytes + ((17 + 17) if traced else (25 if plastic else 0))) + spk * 4
     fl = (st["flush_slots"] * 4 + st["flush_synapses"] * (2 * s_s)) / steps if plastic else 0.0
-    if traced:  # the eager passes of step T - 1 run inside the delivery kernel of step T
-        return {"receive": recv + fl, "update": upd, "flush": 0.0}
-    return {"receive": recv, "update": upd, "flush": fl}
+    # the plasticity work of step T - 1 runs inside the delivery kernel of step T
+    return {"receive": recv + fl, "update": upd, "flush": 0.0}
 
 
 def measured_traffic(workload, kernel):

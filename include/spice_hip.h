@@ -54,11 +54,8 @@ enum spice_flag
 	/* Sum receive() contributions per target attribute as int64 fixed point (units of
 	 * 2^-40) instead of fp32 atomics. Integer addition is associative, so results do not
 	 * depend on atomic order and match the CPU checker's fixed-point mode bit for bit. */
-	SPICE_FLAG_FIXED_POINT = 1u << 0,
-	/* Launch-shape choice (same results): bring queued plastic rows up to date with the
-	 * target-tiled kernel instead of the row-major one (lazy plasticity, i.e. models of the
-	 * per-synapse contract; no effect for neuron-trace synapses, whose plasticity is eager). */
-	SPICE_FLAG_FLUSH_TILED = 1u << 1
+	SPICE_FLAG_FIXED_POINT = 1u << 0
+	/* other bits are rejected (SPICE_ERR_INVALID_ARGUMENT) */
 };
 
 /* Attribute type codes reported by spice_model_info. */
@@ -222,10 +219,10 @@ int spice_hip_run( spice_net * net, int64_t nsteps );
 
 /* Enqueues nsteps steps without graph capture, with HIP events recorded around every kernel
  * on the net's stream, waits, and adds the summed per-kernel device times in milliseconds to
- * kernel_ms[0] (spike delivery; for neuron-trace plasticity including the previous step's
- * plasticity passes, which run in the same kernel), kernel_ms[1] (neuron update) and
- * kernel_ms[2] (lazy-plasticity flush kernel; models of the per-synapse contract only). Same
- * results as spice_hip_run. Used for roofline figures. */
+ * kernel_ms[0] (spike delivery; for plastic models including the previous step's plasticity
+ * work, which runs in the same kernel), kernel_ms[1] (neuron update) and kernel_ms[2]
+ * (always 0: no separate plasticity kernel). Same results as spice_hip_run. Used for roofline
+ * figures. */
 int spice_hip_run_profiled( spice_net * net, int64_t nsteps, double * kernel_ms );
 
 /* Waits for all work of the net. */

@@ -16,7 +16,6 @@ SPICE_ERR_BAD_ALLOC = 2
 SPICE_ERR_RUNTIME = 3
 SPICE_ERR_BAD_CAST = 4
 SPICE_FLAG_FIXED_POINT = 1
-SPICE_FLAG_FLUSH_TILED = 2
 
 
 class SpiceError(RuntimeError):

@@ -30,7 +30,6 @@ class hip_batch final : public batch_iface
 {
 	static constexpr int graph_steps = 32;
 	static constexpr bool plastic = is_plastic_v<Model>;
-	static constexpr bool traced = plastic && has_neuron_traces_v<typename Model::synapse>;
 	// delivery workgroup size: most tiles of a sweep belong to small nets and carry few events,
 	// so more, smaller workgroups shorten the launch's critical path
 	static constexpr int recv_block = SPICE_BATCH_RECV_BLOCK;
@@ -43,7 +42,6 @@ public:
 	    , _owned( std::move( owned ) )
 	    , _s( stream )
 	    , _own_stream( own_stream )
-	    , _lane( plastic && !traced )
 	{
 		if( _nets.empty() ) throw std::invalid_argument( "empty batch" );
 		_fixed = _nets[0]->fixed();
@@ -113,7 +111,6 @@ public:
 	~hip_batch() override
 	{
 		if( _exec ) (void)hipGraphExecDestroy( _exec );
-		_lane.synchronize_quiet();
 		if( _s ) (void)hipStreamSynchronize( _s );
 		if( _s && _own_stream ) (void)hipStreamDestroy( _s );
 	}
@@ -130,7 +127,6 @@ public:
 		if( n - done >= graph_steps )
 		{
 			if( !_exec ) capture();
-			_lane.join_into( _s );
 			for( ; done + graph_steps <= n; done += graph_steps )
 			{
 				SPICE_HIP( hipGraphLaunch( _exec, _s ) );
@@ -150,7 +146,6 @@ public:
 
 	void synchronize() override
 	{
-		_lane.synchronize();
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 	int size() const override { return int( _nets.size() ); }
@@ -165,7 +160,6 @@ private:
 	std::vector<std::unique_ptr<hip_net<Model>>> _owned;
 	hipStream_t _s = nullptr;
 	bool _own_stream = false;
-	side_lane _lane; // lazy plasticity: the flush of step T overlaps step T + 1
 	hipGraphExec_t _exec = nullptr;
 	bool _fixed = false;
 	std::int64_t _step = 0;
@@ -182,35 +176,23 @@ private:
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
-	// Same schedule as hip_net::launch_step: the side work of step t overlaps step t + 1.
-	void launch_step( std::int64_t t, int fresh = 2 )
+	// Same schedule as hip_net::launch_step: delivery with step t - 1's plasticity, update.
+	void launch_step( std::int64_t t )
 	{
 		if( _fixed )
-			launch_step_t<true>( t, fresh );
+			launch_step_t<true>( t );
 		else
-			launch_step_t<false>( t, fresh );
+			launch_step_t<false>( t );
 	}
 
 	template <bool F>
-	void launch_step_t( std::int64_t t, int fresh )
+	void launch_step_t( std::int64_t t )
 	{
-		constexpr bool lazy = plastic && !traced;
 		dview const * v = _views[t & ( CLOCK_SLOTS - 1 )].data();
-		if constexpr( lazy )
-			if( fresh >= 2 ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
-		// eager plasticity: the nets' step t - 1 passes run in the delivery kernel's extra workgroups
-		unsigned const grid = unsigned( _grid_tiles + ( traced ? _nflush : 0 ) );
+		unsigned const grid = unsigned( _grid_tiles + ( plastic ? _nflush : 0 ) );
 		SPICE_LAUNCH( k_batch_receive<Model, F, recv_block><<<grid, recv_block, 0, _s>>>( v, _tile_net.data(), _tile_local.data(), _ntiles, _grid_tiles,
 		                                                                                    _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
 		SPICE_LAUNCH( k_batch_update<Model, F><<<_nblocks, 256, 0, _s>>>( v, _block_net.data(), _block_local.data() ) );
-		if constexpr( lazy )
-		{
-			hipStream_t const ls = _lane.s[t & 1];
-			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
-			SPICE_HIP( hipStreamWaitEvent( ls, _lane.up, 0 ) );
-			SPICE_LAUNCH( k_batch_flush<Model, F><<<_nflush, 256, 0, ls>>>( v, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
-			SPICE_HIP( hipEventRecord( _lane.done[t & 1], ls ) );
-		}
 	}
 
 	void capture()
@@ -219,8 +201,7 @@ private:
 		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
 		try
 		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step( i, i );
-			_lane.join_into( _s );
+			for( int i = 0; i < graph_steps; i++ ) launch_step( i );
 		}
 		catch( ... )
 		{

@@ -458,6 +458,7 @@ static job_spec job_spec_of( spice_job_desc const & j )
 	require( j.dt > 0.0f, "dt must be positive" );
 	require( j.net.delay >= 1, "delay must be >= 1" );
 	require( j.steps >= 1, "a job runs at least one step" );
+	require( ( j.flags & ~std::uint32_t( SPICE_FLAG_FIXED_POINT ) ) == 0, "unknown flags" );
 	util::layout const l = make_layout( j.net.pops, j.net.npops, j.net.conns, j.net.nconns );
 	require( l.size() >= 1, "a job needs at least one neuron" );
 	job_spec s;

@@ -27,73 +27,6 @@ inline void check( hipError_t e, char const * what )
 		::spice::hipx::check( hipGetLastError(), "kernel launch" );   \
 	} while( 0 )
 
-// Side streams and events for work that overlaps the following steps: the lazy-plasticity
-// flush of step T runs on stream s[T & 1] beside the delivery and update of steps T + 1 (and
-// T + 2), so the flushes of two consecutive steps may run at the same time (their rows are
-// disjoint). Inactive if `on` is false.
-struct side_lane
-{
-	hipStream_t s[2] = { nullptr, nullptr };
-	hipEvent_t up = nullptr, join = nullptr, done[2] = { nullptr, nullptr };
-
-	explicit side_lane( bool on )
-	{
-		if( !on ) return;
-		try
-		{
-			for( hipStream_t * x : { &s[0], &s[1] } ) check( hipStreamCreateWithFlags( x, hipStreamNonBlocking ), "hipStreamCreateWithFlags" );
-			for( hipEvent_t * e : { &up, &join, &done[0], &done[1] } )
-				check( hipEventCreateWithFlags( e, hipEventDisableTiming ), "hipEventCreateWithFlags" );
-		}
-		catch( ... )
-		{
-			release();
-			throw;
-		}
-	}
-	~side_lane() { release(); }
-	side_lane( side_lane const & ) = delete;
-	side_lane & operator=( side_lane const & ) = delete;
-
-	bool active() const { return s[0] != nullptr; }
-
-	// main stream waits for everything enqueued on the side streams so far
-	void join_into( hipStream_t main )
-	{
-		if( !active() ) return;
-		for( hipStream_t x : s )
-		{
-			check( hipEventRecord( join, x ), "hipEventRecord" );
-			check( hipStreamWaitEvent( main, join, 0 ), "hipStreamWaitEvent" );
-		}
-	}
-
-	void synchronize()
-	{
-		for( hipStream_t x : s )
-			if( x ) check( hipStreamSynchronize( x ), "hipStreamSynchronize" );
-	}
-	void synchronize_quiet() noexcept
-	{
-		for( hipStream_t x : s )
-			if( x ) (void)hipStreamSynchronize( x );
-	}
-
-private:
-	void release()
-	{
-		for( hipStream_t x : s )
-			if( x ) (void)hipStreamSynchronize( x );
-		for( hipEvent_t * e : { &up, &join, &done[0], &done[1] } )
-			if( *e ) (void)hipEventDestroy( *e );
-		for( hipStream_t & x : s )
-		{
-			if( x ) (void)hipStreamDestroy( x );
-			x = nullptr;
-		}
-	}
-};
-
 // Owning device allocation (reference counterpart: dbuffer, spice/cuda/util/dbuffer.h:17-90).
 template <typename T>
 class dbuf

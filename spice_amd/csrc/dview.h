@@ -19,16 +19,18 @@
 //   last_spk int32[N]    latest spike step
 //   synced   int32[N]    row s is up to date through this step
 //   pend     int32[N]    row s has an undelivered pre event at synced+1
-//   fl_*     int32[2][N] rows whose history window is full this step, with their
-//                        (synced, pend) at that time; lists by step parity, counters [4]
-//                        by step mod 4
+//   fl_*     int32[2][N] rows queued at this step (spike arrived, or history window full),
+//                        with their (synced, pend) at that time; lists by step parity,
+//                        counters [4] by step mod 4
 // eager plasticity (neuron-trace synapses), all [2][N] arrays by step parity:
 //   xtr/ytr  f32[2][N]   neuron-local pre/post traces after step T
 //   fl8      u8[2][N]    bit 0: spiked at T, bit 1: its spike arrives at T, bit 2: at T + 1
 //   fl_id    int32[2][N] rows whose spike arrived at T (and does not arrive at T + 1)
 //   po_id    int32[2][N] neurons that spiked at T and have incoming plastic synapses
 //   tp_ptr   int64[N+1], tp_ent {src, offset in src's row}[P]: the P plastic synapses by target
-//   caught   int64       step whose deferred events (rows arriving next) a sync already applied
+// both:
+//   caught   int64       step whose plasticity work (and, eager, deferred events of the rows
+//                        arriving next) a synchronization already did
 //   slog_ids int32[cap]  spike log (served jobs): ids of steps t0 .. t0+nsteps-1, step by step
 //   slog_off u64[nsteps] offset of step t0+k in slog_ids
 #include <cstdint>
@@ -123,6 +125,7 @@ struct dview
 
 	ctl_t * ctl;      // this step's clock
 	ctl_t * ctl_next; // the next step's clock (written by k_update)
+	ctl_t * ctl_prev; // the previous step's clock (its dt for the lazy flush of that step)
 	unsigned long long * bstats; // [nblocks_max * NSTAT]
 
 	std::int32_t * pivots; // [N][ntiles + 1]: offset in row r of its first target >= t * tile_w

@@ -394,8 +394,6 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 #define SPICE_BATCH_RECV_BLOCK 256
 #endif
 constexpr int TILE_MAX = 1024; // targets per tile (LDS accumulators)
-constexpr int TBS = 1024;      // threads per tile workgroup of the tiled flush (16 waves)
-constexpr int SPK_CHUNK = TBS; // rows (flushed rows) staged in LDS per pass
 // Delivery workgroups have 256 or 1024 threads (a launch shape): they stage that many spikes
 // per pass in LDS.
 
@@ -709,21 +707,29 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 
 template <typename Model, bool FIXED>
 __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t blk, std::int64_t nblk, std::int64_t slot0 );
-
-// Eager plasticity (neuron-trace synapses) runs inside the delivery kernel: the pre and post
-// passes of step T - 1 touch no row that step T delivers, so workgroups beyond the delivery
-// ones apply them while delivery runs, and the next kernel in stream order (the update of
-// step T, then the delivery of step T + 1 with the passes of step T) sees them done. Skipped
-// when a synchronization has applied them already (v.caught = T - 1).
 template <typename Model, bool FIXED>
-__device__ inline void stdp_prev_block( dview const & v, std::int64_t blk, std::int64_t nblk )
+__device__ inline void flush_block( dview const & v, std::int64_t T, std::int64_t blk, std::int64_t nblk, std::int64_t slot0 );
+
+// Plasticity runs inside the delivery kernel: the plasticity work of step T - 1 (eager pre
+// and post passes, or the lazy flush of the rows queued at T - 1) touches no row that step T
+// delivers, so workgroups beyond the delivery ones do it while delivery runs, and the next
+// kernel in stream order (the update of step T, then the delivery of step T + 1 with step T's
+// plasticity) sees it done. Skipped when a synchronization has done it already
+// (v.caught = T - 1).
+template <typename Model, bool FIXED>
+__device__ inline void plasticity_prev_block( dview const & v, std::int64_t blk, std::int64_t nblk )
 {
 	std::int64_t const T = v.ctl->step;
-	if( T >= 1 && *v.caught != T - 1 ) stdp_block<Model, FIXED>( v, T - 1, blk, nblk, std::int64_t( v.ntiles ) * v.rparts );
+	if( T < 1 || *v.caught == T - 1 ) return;
+	std::int64_t const slot0 = std::int64_t( v.ntiles ) * v.rparts;
+	if constexpr( has_neuron_traces_v<typename Model::synapse> )
+		stdp_block<Model, FIXED>( v, T - 1, blk, nblk, slot0 );
+	else
+		flush_block<Model, FIXED>( v, T - 1, blk, nblk, slot0 );
 }
 
-// Workgroups [0, grecv) deliver (work items round robin); eager plasticity models give the
-// workgroups [grecv, gridDim.x) to step T - 1's plasticity passes.
+// Workgroups [0, grecv) deliver (work items round robin); plastic models give the workgroups
+// [grecv, gridDim.x) to step T - 1's plasticity work.
 template <typename Model, bool FIXED, int BS>
 __global__ __launch_bounds__( BS ) void k_receive_tiled( dview v, std::int32_t grecv )
 {
@@ -732,13 +738,13 @@ __global__ __launch_bounds__( BS ) void k_receive_tiled( dview v, std::int32_t g
 		std::int32_t const nwork = v.ntiles * v.rparts;
 		for( std::int32_t w = blockIdx.x; w < nwork; w += grecv ) receive_tile<Model, FIXED, BS>( v, w );
 	}
-	else if constexpr( is_plastic_v<Model> && has_neuron_traces_v<typename Model::synapse> )
-		stdp_prev_block<Model, FIXED>( v, blockIdx.x - grecv, gridDim.x - grecv );
+	else if constexpr( is_plastic_v<Model> )
+		plasticity_prev_block<Model, FIXED>( v, blockIdx.x - grecv, gridDim.x - grecv );
 }
 
 // Batched delivery: global work item gt belongs to net tile_net[gt] (local work item
-// tile_local[gt] = tile * rparts + part); the net's view is read from device memory. Eager
-// plasticity: workgroup grecv + b runs plasticity block block_local[b] of the block_count[b]
+// tile_local[gt] = tile * rparts + part); the net's view is read from device memory. Plastic
+// models: workgroup grecv + b runs plasticity block block_local[b] of the block_count[b]
 // blocks of net block_net[b].
 template <typename Model, bool FIXED, int BS>
 __global__ __launch_bounds__( BS ) void k_batch_receive( dview const * views, std::int32_t const * tile_net, std::int32_t const * tile_local,
@@ -750,102 +756,14 @@ __global__ __launch_bounds__( BS ) void k_batch_receive( dview const * views, st
 		for( std::int32_t gt = blockIdx.x; gt < ntiles; gt += grecv )
 			receive_tile<Model, FIXED, BS>( views[tile_net[gt]], tile_local[gt] );
 	}
-	else if constexpr( is_plastic_v<Model> && has_neuron_traces_v<typename Model::synapse> )
+	else if constexpr( is_plastic_v<Model> )
 	{
 		std::int32_t const b = blockIdx.x - grecv;
-		stdp_prev_block<Model, FIXED>( views[block_net[b]], block_local[b], block_count[b] );
+		plasticity_prev_block<Model, FIXED>( views[block_net[b]], block_local[b], block_count[b] );
 	}
 }
 
 
-
-// Brings the rows queued by this step's neuron update up to date (lazy plasticity of the
-// per-synapse contract), one workgroup per target tile: the queued rows' plastic segments
-// inside the tile are walked as one flat index space.
-template <typename Model, bool FIXED>
-__global__ __launch_bounds__( TBS ) void k_flush_tiled( dview v )
-{
-	using S = typename Model::synapse;
-	static_assert( !has_neuron_traces_v<S>, "neuron-trace synapses use the eager passes" );
-	constexpr int U = 2;
-	__shared__ std::int64_t seg_beg[SPK_CHUNK];
-	__shared__ std::int32_t seg_pre[SPK_CHUNK + 1];
-	__shared__ std::int32_t seg_src[SPK_CHUNK];
-	__shared__ std::int32_t seg_synced[SPK_CHUNK];
-	__shared__ std::int32_t seg_pend[SPK_CHUNK];
-	__shared__ std::int32_t wsum[TBS / 64];
-
-	std::int64_t const T = v.ctl->step;
-	std::int32_t const nrows = std::int32_t( v.fl_count[T & 3] );
-	std::int64_t const fb = ( T & 1 ) * std::int64_t( v.N ); // this step's flush list
-	float const dt = step_dt( v );
-	snn_info const info = info_of( v );
-	int const tid = threadIdx.x;
-	unsigned long long touched = 0, slots = 0;
-	if( nrows > 0 )
-		for( std::int32_t tile = blockIdx.x; tile < v.ntiles; tile += gridDim.x )
-		{
-			for( std::int32_t c0 = 0; c0 < nrows; c0 += SPK_CHUNK )
-			{
-				std::int32_t const cnt = ( nrows - c0 ) < SPK_CHUNK ? ( nrows - c0 ) : SPK_CHUNK;
-				std::int32_t len = 0;
-				if( tid < cnt )
-				{
-					std::int32_t const r = v.fl_id[fb + c0 + tid];
-					std::int32_t const * pv = v.pivots + std::int64_t( r ) * ( v.ntiles + 1 ) + tile;
-					std::int32_t a = pv[0], b = pv[1];
-					std::int32_t const pa = v.pl_lo[r], pb = v.pl_hi[r];
-					a = a > pa ? a : pa;
-					b = b < pb ? b : pb;
-					len = b > a ? b - a : 0;
-					seg_beg[tid] = v.row_ptr[r] + a;
-					seg_src[tid] = r;
-					seg_synced[tid] = v.fl_synced[fb + c0 + tid];
-					seg_pend[tid] = v.fl_pend[fb + c0 + tid];
-				}
-				std::int32_t const pre = block_exclusive_scan( len, wsum, &seg_pre[SPK_CHUNK] );
-				if( tid < cnt ) seg_pre[tid] = pre;
-				__syncthreads();
-				std::int32_t const total = seg_pre[SPK_CHUNK];
-				for( std::int32_t f0 = 0; f0 < total; f0 += TBS * U )
-				{
-					std::int32_t kk[U];
-					std::int64_t e[U];
-					std::int32_t j[U];
-					bool ok[U];
-#pragma unroll
-					for( int u = 0; u < U; u++ )
-					{
-						std::int32_t const f = f0 + tid + TBS * u;
-						ok[u] = f < total;
-						kk[u] = segment_of( seg_pre, cnt, f );
-						e[u] = seg_beg[kk[u]] + ( f - seg_pre[kk[u]] );
-						j[u] = ok[u] ? v.col[e[u]] : 0;
-					}
-#pragma unroll
-					for( int u = 0; u < U; u++ )
-					{
-						if( !ok[u] ) continue;
-						std::int32_t const s = seg_src[kk[u]];
-						if( !S::plastic( s, j[u], info ) ) continue;
-						std::int32_t const synced = seg_synced[kk[u]], pend = seg_pend[kk[u]];
-						typename S::tuple_t sy;
-						load_tuple<S>( sy, v.sa, e[u] );
-						auto bak = make_backend<FIXED>(
-						    v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
-						reg_iter<typename S::tuple_t> itr{ sy, std::uint32_t( e[u] ) };
-						replay<S>( itr, v, j[u], std::int32_t( T ), synced, pend, dt, info, bak );
-						store_tuple<S>( sy, v.sa, e[u] );
-						touched++;
-					}
-				}
-				if( tid == 0 ) slots += std::uint64_t( total );
-				__syncthreads();
-			}
-		}
-	block_add_stats<3>( v, blockIdx.x + ( T & 1 ) * gridDim.x, { touched, slots, blockIdx.x == 0 && tid == 0 ? std::uint64_t( nrows ) : 0ull },
-	                    { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
-}
 
 template <typename Model, bool FIXED>
 __device__ inline void update_block( dview const & v, std::int64_t blk )
@@ -1209,19 +1127,19 @@ __global__ __launch_bounds__( 256 ) void k_stdp( dview v, std::int64_t T, std::i
 	stdp_block<Model, FIXED>( v, T, blockIdx.x, gridDim.x, slot0 );
 }
 
-// Row-major flush of the rows whose history window is full (lazy plasticity of the
+// Row-major flush of the rows queued by the update of step T (lazy plasticity of the
 // per-synapse contract): work item = (row, part); waves of workgroups blk, blk + nblk, ...
-// (of this net) take items round robin.
+// (of this net) take items round robin. v is a view of step T + 1 (its ctl_prev is step T's
+// clock). Statistics go to slots slot0 + blk.
 template <typename Model, bool FIXED>
-__device__ inline void flush_block( dview const & v, std::int64_t blk, std::int64_t nblk )
+__device__ inline void flush_block( dview const & v, std::int64_t T, std::int64_t blk, std::int64_t nblk, std::int64_t slot0 )
 {
 	using S = typename Model::synapse;
 	static_assert( !has_neuron_traces_v<S>, "neuron-trace synapses use the eager passes" );
 	constexpr int U = 4; // slots per lane per iteration: independent loads in flight
-	std::int64_t const T = v.ctl->step;
 	std::int64_t const nitems = std::int64_t( v.fl_count[T & 3] ) * v.fparts;
-	std::int64_t const fb = ( T & 1 ) * std::int64_t( v.N ); // this step's flush list
-	float const dt = step_dt( v );
+	std::int64_t const fb = ( T & 1 ) * std::int64_t( v.N ); // step T's flush list
+	float const dt = v.DT - v.ctl_prev->c;                    // step T's compensated dt
 	snn_info const info = info_of( v );
 	int const lane = lane_id();
 	std::int64_t const wave0 = uniform64( ( blk * blockDim.x + threadIdx.x ) >> 6 );
@@ -1273,24 +1191,16 @@ __device__ inline void flush_block( dview const & v, std::int64_t blk, std::int6
 			if( p == 0 ) rows++;
 		}
 	}
-	// the flushes of two consecutive steps may run at once: one set of slots per step parity
-	block_add_stats<3>( v, blk + ( T & 1 ) * nblk, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	block_add_stats<3>( v, slot0 + blk, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
+// The lazy flush of step `T` on its own (a synchronization before a state export runs the
+// flush of the last completed step that the next delivery kernel would otherwise run); v is
+// a view of step T + 1.
 template <typename Model, bool FIXED>
-__global__ __launch_bounds__( 256 ) void k_flush( dview v )
+__global__ __launch_bounds__( 256 ) void k_flush( dview v, std::int64_t T, std::int64_t slot0 )
 {
-	flush_block<Model, FIXED>( v, blockIdx.x, gridDim.x );
-}
-
-// Batched flush: workgroup b is flush block block_local[b] of the block_count[b] blocks of
-// net block_net[b].
-template <typename Model, bool FIXED>
-__global__ __launch_bounds__( 256 ) void k_batch_flush(
-    dview const * views, std::int32_t const * block_net, std::int32_t const * block_local, std::int32_t const * block_count )
-{
-	std::int32_t const b = blockIdx.x;
-	flush_block<Model, FIXED>( views[block_net[b]], block_local[b], block_count[b] );
+	flush_block<Model, FIXED>( v, T, blockIdx.x, gridDim.x, slot0 );
 }
 
 // Opens the spike-log window of every net of a batch: the next nsteps steps, starting at the

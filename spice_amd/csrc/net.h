@@ -109,9 +109,9 @@ public:
 	    : _p( p )
 	    , _g( std::move( g ) )
 	    , _s( stream )
-	    , _lane( plastic && !traced )
 	{
 		if( !( p.dt > 0.0f ) ) throw std::invalid_argument( "dt must be positive" );
+		if( p.flags & ~std::uint32_t( SPICE_FLAG_FIXED_POINT ) ) throw std::invalid_argument( "unknown flags" );
 		if( p.delay < 1 ) throw std::invalid_argument( "delay must be >= 1" );
 		if( plastic && p.delay > HISTORY - 1 )
 			throw std::invalid_argument( "plastic models support delays of at most 255 steps" );
@@ -124,7 +124,6 @@ public:
 	~hip_net() override
 	{
 		if( _exec ) (void)hipGraphExecDestroy( _exec );
-		_lane.synchronize_quiet();
 		if( _s ) (void)hipStreamSynchronize( _s );
 		if( _s ) (void)hipStreamDestroy( _s );
 	}
@@ -154,7 +153,6 @@ public:
 		if( n - done >= graph_steps )
 		{
 			if( !_exec ) capture();
-			_lane.join_into( _s ); // flushes of eager steps before the graph
 			for( ; done + graph_steps <= n; done += graph_steps )
 			{
 				SPICE_HIP( hipGraphLaunch( _exec, _s ) );
@@ -199,7 +197,7 @@ public:
 					hipEvent_t * e = &ev[i * ev_per_step];
 					for( int q = 0; q < 3; q++ )
 					{
-						if( q == 2 && !( plastic && !traced ) ) continue;
+						if( q == 2 ) continue; // plasticity runs inside the delivery kernel
 						float ms = 0;
 						SPICE_HIP( hipEventElapsedTime( &ms, e[q], e[q + 1] ) );
 						kernel_ms[q] += ms;
@@ -236,11 +234,7 @@ public:
 		}
 	}
 
-	void synchronize() override
-	{
-		_lane.synchronize();
-		SPICE_HIP( hipStreamSynchronize( _s ) );
-	}
+	void synchronize() override { SPICE_HIP( hipStreamSynchronize( _s ) ); }
 
 	// ---- queries ----
 	net_params const & params() const override { return _p; }
@@ -268,6 +262,7 @@ public:
 		{
 			SPICE_HIP( hipMemset( _hring.data(), 0, _hring.bytes() ) );
 			SPICE_HIP( hipMemset( _fl_count.data(), 0, _fl_count.bytes() ) );
+			reset_caught();
 			if constexpr( traced )
 				reset_eager();
 			else
@@ -519,7 +514,6 @@ private:
 	net_params _p;
 	csr_dev _g;
 	hipStream_t _s = nullptr;
-	side_lane _lane; // lazy plasticity: the flush of step T overlaps step T + 1
 	hipGraphExec_t _exec = nullptr;
 
 	dbuf<char> _na[MAXA], _acc[MAXA], _sa[MAXA];
@@ -535,7 +529,8 @@ private:
 	dbuf<std::uint8_t> _fl8;                   // eager: [2][N] step flags
 	dbuf<std::int32_t> _po_id;                 // eager: [2][N] spiking targets
 	dbuf<std::uint32_t> _po_count;             // eager: [4]
-	dbuf<std::int64_t> _tp_ptr, _caught;       // eager: by-target index offsets; synced step
+	dbuf<std::int64_t> _tp_ptr;                // eager: by-target index offsets
+	dbuf<std::int64_t> _caught;                // plastic: step whose plasticity a synchronization did
 	dbuf<tp_entry> _tp_ent;                    // eager: by-target index entries
 	std::int64_t _caught_host = -1;
 	std::int64_t _tp_total = 0, _tp_targets = 0;
@@ -546,7 +541,7 @@ private:
 	dview _vs[CLOCK_SLOTS]{};
 
 	int _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
-	int _grid_plast = 0; // eager plasticity workgroups of the delivery kernel
+	int _grid_plast = 0; // plasticity workgroups of the delivery kernel
 	launch_cfg _lc{};
 	dbuf<std::int32_t> _pivots;
 	std::int64_t _step = 0;
@@ -600,8 +595,10 @@ private:
 			_pl_hi.resize( n );
 			_fl_id.resize( 2 * n ); // lists of steps of both parities
 			_fl_count.resize( CLOCK_SLOTS );
+			_caught.resize( 1 );
 			SPICE_HIP( hipMemsetAsync( _hring.data(), 0, _hring.bytes(), _s ) );
 			SPICE_HIP( hipMemsetAsync( _fl_count.data(), 0, _fl_count.bytes(), _s ) );
+			reset_caught();
 			if constexpr( traced )
 			{
 				_xtr.resize( 2 * n );
@@ -609,7 +606,6 @@ private:
 				_fl8.resize( 2 * n );
 				_po_id.resize( 2 * n );
 				_po_count.resize( CLOCK_SLOTS );
-				_caught.resize( 1 );
 				_tp_ptr.resize( n + 1 );
 				reset_eager();
 			}
@@ -675,13 +671,17 @@ private:
 		apply_launch( launch_cfg{} );
 	}
 
-	// Eager plasticity state at a clock (re)start: zero traces and flags, no pending lists,
-	// nothing synchronized.
+	// Eager plasticity state at a clock (re)start: zero traces and flags, no pending lists.
 	void reset_eager()
 	{
 		for( auto * b : { &_xtr, &_ytr } ) SPICE_HIP( hipMemsetAsync( b->data(), 0, b->bytes(), _s ) );
 		SPICE_HIP( hipMemsetAsync( _fl8.data(), 0, _fl8.bytes(), _s ) );
 		SPICE_HIP( hipMemsetAsync( _po_count.data(), 0, _po_count.bytes(), _s ) );
+	}
+
+	// Nothing synchronized yet (plastic models).
+	void reset_caught()
+	{
 		std::int64_t const none = -1;
 		SPICE_HIP( hipMemcpyAsync( _caught.data(), &none, 8, hipMemcpyHostToDevice, _s ) );
 		SPICE_HIP( hipStreamSynchronize( _s ) ); // `none` is a host temporary
@@ -772,7 +772,7 @@ private:
 		v.rparts = c.recv_parts;
 		_grid_flush = c.flush_grid;
 		_grid_tiles = std::min( v.ntiles * v.rparts, ( 16 * 1024 / c.recv_block ) * cus );
-		_grid_plast = traced ? std::max( 1, _grid_flush * 256 / c.recv_block ) : 0; // the same waves as flush_grid
+		_grid_plast = plastic ? std::max( 1, _grid_flush * 256 / c.recv_block ) : 0; // the same waves as flush_grid
 		_lc = c;
 		// statistics slots: one per workgroup of the widest step kernel (receive: per work item;
 		// eager plasticity: after those, one per plasticity workgroup or synchronization block)
@@ -800,6 +800,7 @@ private:
 			_vs[p] = _v;
 			_vs[p].ctl = _ctl.data() + p;
 			_vs[p].ctl_next = _ctl.data() + ( ( p + 1 ) & ( CLOCK_SLOTS - 1 ) );
+			_vs[p].ctl_prev = _ctl.data() + ( ( p + CLOCK_SLOTS - 1 ) & ( CLOCK_SLOTS - 1 ) );
 		}
 		if( retile ) SPICE_LAUNCH( k_pivots<<<unsigned( ( n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
 		if( _exec )
@@ -820,53 +821,24 @@ private:
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
-	// Enqueues the kernels of step `t` (t mod 4 selects the clock slot). Lazy plasticity: the
-	// flush of step t goes to the side lane after step t's update and overlaps step t + 1;
-	// step t's delivery first waits for the flush of step t - 2. `fresh`: steps of the graph
-	// being captured before step t (2: two or more, or not capturing) -- flushes recorded
-	// before the capture finished before the graph started, and are not waited for. Eager
-	// plasticity: step t - 1's passes run inside step t's delivery kernel.
-	void launch_step( std::int64_t t, int fresh = 2 )
+	// Enqueues the kernels of step `t` (t mod 4 selects the clock slot): delivery (plastic
+	// models: with step t - 1's plasticity work in the same kernel), then the neuron update.
+	void launch_step( std::int64_t t )
 	{
 		dview const & v = _vs[t & ( CLOCK_SLOTS - 1 )];
 		if( fixed() )
-			launch_step_t<true>( v, t, fresh );
-		else
-			launch_step_t<false>( v, t, fresh );
-	}
-
-	template <bool F>
-	void launch_step_t( dview const & v, std::int64_t t, int fresh )
-	{
-		constexpr bool lazy = plastic && !traced;
-		if constexpr( lazy )
-			if( fresh >= 2 ) SPICE_HIP( hipStreamWaitEvent( _s, _lane.done[t & 1], 0 ) );
-		launch_receive<F>( v );
-		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
-		if constexpr( lazy )
 		{
-			// the flushes of consecutive steps touch different rows and alternate between two
-			// side streams
-			hipStream_t const ls = _lane.s[t & 1];
-			SPICE_HIP( hipEventRecord( _lane.up, _s ) );
-			SPICE_HIP( hipStreamWaitEvent( ls, _lane.up, 0 ) );
-			launch_flush<F>( v, ls );
-			SPICE_HIP( hipEventRecord( _lane.done[t & 1], ls ) );
+			launch_receive<true>( v );
+			SPICE_LAUNCH( k_update<Model, true><<<_grid_update, 256, 0, _s>>>( v ) );
+		}
+		else
+		{
+			launch_receive<false>( v );
+			SPICE_LAUNCH( k_update<Model, false><<<_grid_update, 256, 0, _s>>>( v ) );
 		}
 	}
 
-	// SPICE_FLAG_FLUSH_TILED selects the target-tiled lazy flush
-	template <bool F>
-	void launch_flush( dview const & v, hipStream_t st )
-	{
-		static_assert( !traced, "neuron-trace synapses use the eager passes" );
-		if( _p.flags & SPICE_FLAG_FLUSH_TILED )
-			SPICE_LAUNCH( k_flush_tiled<Model, F><<<_grid_tiles, TBS, 0, st>>>( v ) );
-		else
-			SPICE_LAUNCH( k_flush<Model, F><<<_grid_flush, 256, 0, st>>>( v ) );
-	}
-
-	// delivery workgroups, plus (eager plasticity) the previous step's plasticity workgroups
+	// delivery workgroups, plus (plastic models) the previous step's plasticity workgroups
 	template <bool F>
 	void launch_receive( dview const & v )
 	{
@@ -887,7 +859,6 @@ private:
 		SPICE_HIP( hipEventRecord( e[1], _s ) );
 		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[2], _s ) );
-		if constexpr( plastic && !traced ) launch_flush<F>( v, _s );
 		SPICE_HIP( hipEventRecord( e[3], _s ) );
 	}
 
@@ -897,8 +868,7 @@ private:
 		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
 		try
 		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step( i, i ); // clock slots 0, 1, 2, 3, 0, ...
-			_lane.join_into( _s );                                          // the graph ends with its flushes
+			for( int i = 0; i < graph_steps; i++ ) launch_step( i ); // clock slots 0, 1, 2, 3, 0, ...
 		}
 		catch( ... )
 		{
@@ -916,32 +886,36 @@ private:
 	void sync_synapses()
 	{
 		synchronize();
-		if constexpr( traced )
+		if constexpr( plastic )
 		{
-			// the last step's plasticity passes and its events of the rows arriving next (once:
-			// the next delivery kernel then skips both)
 			if( _step > 0 && _caught_host != _step - 1 )
 			{
-				dview const & v = _vs[( _step - 1 ) & ( CLOCK_SLOTS - 1 )];
-				std::int64_t const slot0 = std::int64_t( v.ntiles ) * v.rparts;
-				if( fixed() )
-					SPICE_LAUNCH( k_stdp<Model, true><<<_grid_flush, 256, 0, _s>>>( v, _step - 1, slot0 ) );
-				else
-					SPICE_LAUNCH( k_stdp<Model, false><<<_grid_flush, 256, 0, _s>>>( v, _step - 1, slot0 ) );
-				SPICE_LAUNCH( k_sync_eager<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, _step - 1 ) );
+				// the last step's plasticity work, which the next delivery kernel would run, then
+				// (eager) its events of the rows arriving next or (lazy) every row's replay up to it;
+				// recorded in `caught` so the next delivery kernel skips the former
 				std::int64_t const now = _step - 1;
+				dview const & v = _vs[_step & ( CLOCK_SLOTS - 1 )]; // a view of step now + 1
+				std::int64_t const slot0 = std::int64_t( v.ntiles ) * v.rparts;
+				if constexpr( traced )
+				{
+					if( fixed() )
+						SPICE_LAUNCH( k_stdp<Model, true><<<_grid_flush, 256, 0, _s>>>( v, now, slot0 ) );
+					else
+						SPICE_LAUNCH( k_stdp<Model, false><<<_grid_flush, 256, 0, _s>>>( v, now, slot0 ) );
+					SPICE_LAUNCH( k_sync_eager<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, now ) );
+				}
+				else
+				{
+					if( fixed() )
+						SPICE_LAUNCH( k_flush<Model, true><<<_grid_flush, 256, 0, _s>>>( v, now, slot0 ) );
+					else
+						SPICE_LAUNCH( k_flush<Model, false><<<_grid_flush, 256, 0, _s>>>( v, now, slot0 ) );
+					// dt of the last completed step (the step that moved the clock to its state)
+					SPICE_LAUNCH( k_sync_all<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, std::int32_t( now ), _last_dt() ) );
+				}
 				SPICE_HIP( hipMemcpyAsync( _caught.data(), &now, 8, hipMemcpyHostToDevice, _s ) );
 				SPICE_HIP( hipStreamSynchronize( _s ) ); // `now` is a host temporary
 				_caught_host = now;
-			}
-		}
-		else if constexpr( plastic )
-		{
-			if( _step > 0 )
-			{
-				// dt of the last completed step (the step that moved the clock to its state)
-				float const dt_last = _last_dt();
-				SPICE_LAUNCH( k_sync_all<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, std::int32_t( _step - 1 ), dt_last ) );
 			}
 		}
 		synchronize();

@@ -195,8 +195,8 @@ class Snn:
     def run_profiled(self, nsteps):
         """Runs nsteps eagerly with HIP events around each kernel; returns summed device ms
         per kernel: {"receive": ..., "update": ..., "flush": ...} ("receive" includes the
-        previous step's plasticity passes for neuron-trace synapses; "flush" is the lazy
-        flush kernel of the per-synapse contract, 0 otherwise)."""
+        previous step's plasticity work of plastic models, which runs in the same kernel;
+        "flush" is 0)."""
         ms = (_c.c_double * 3)(0.0, 0.0, 0.0)
         _lib.check(_lib.lib().spice_hip_run_profiled(self._h, int(nsteps), ms))
         return {"receive": ms[0], "update": ms[1], "flush": ms[2]}

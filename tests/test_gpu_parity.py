@@ -186,6 +186,13 @@ def test_plastic_delay_limit(gpu):
     Snn("brunel_3pop", Layout(*three_pop_layout(500)), 1e-4, 1000).close()  # static models: any delay
 
 
+def test_unknown_flags_are_rejected(gpu):
+    """Only SPICE_FLAG_FIXED_POINT is a flag; other bits are an invalid argument."""
+    with pytest.raises(ValueError):
+        Snn("brunel_3pop", Layout(*three_pop_layout(500)), 1e-4, 15, flags=2)
+    Snn("brunel_3pop", Layout(*three_pop_layout(500)), 1e-4, 15, flags=1).close()
+
+
 def test_launch_shapes_do_not_change_results(gpu):
     """Delivery tile width, flush work-item size and flush grid changed between steps (fixed
     point, plastic model): state, weights and counters equal the default-shaped run."""

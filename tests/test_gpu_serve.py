@@ -134,3 +134,5 @@ def test_invalid_jobs_are_rejected_at_submit(gpu):
             sv.submit("brunel_3pop", Layout(*three_pop_layout(500)), DT, 10, delay=0)
         with pytest.raises(ValueError):
             sv.submit("brunel_3pop_stdp", Layout(*three_pop_layout(500)), DT, 10, delay=256)  # plastic delay limit
+        with pytest.raises(ValueError):
+            sv.submit("brunel_3pop", Layout(*three_pop_layout(500)), DT, 10, flags=2)  # unknown flag bit
