@@ -320,21 +320,40 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
     }
 
 
+def host_cpu():
+    """Model name of the host CPU (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(model, lay, delay, seed, net, steps):
-    """Times the CPU checker (single thread) on `steps` steps of the same network: the graph
-    is copied from the GPU, state starts from the model's init (as the GPU run did)."""
+    """Times the CPU checker (single thread, the process pinned to one core while it runs) on
+    `steps` steps of the same network: the graph is copied from the GPU, state starts from the
+    model's init (as the GPU run did). Returns (s per step, init s, core)."""
     from oracle import oracle as O
     rp, col = net.csr()
     t0 = time.perf_counter()
     h = O.CpuSnn.from_csr(model, rp, col, net.width(), 1e-4, delay, seed=seed)
     t_init = time.perf_counter() - t0
     del rp, col
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        h.step()
-    dt_wall = time.perf_counter() - t0
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            h.step()
+        dt_wall = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, old)
     h.close()
-    return dt_wall / steps, t_init
+    return dt_wall / steps, t_init, core
 
 
 def main():
@@ -346,7 +365,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=1_000)
     ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU baseline (0 = skip); ~0.55 s each at 1e9 synapses")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--flags", type=int, default=0, help="spice_flag launch-shape bits (see include/spice_hip.h)")
+    ap.add_argument("--flags", type=int, default=0, help="spice_flag bits (1: int64 fixed-point accumulation; see include/spice_hip.h)")
     ap.add_argument("--batch-nets", type=int, default=512, help="nets of the batched sweep (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-seq-stride", type=int, default=8)
@@ -419,11 +438,12 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0:
-        s_per_step, t_init = cpu_baseline(model, (pops, conns), delay, args.seed, net, args.cpu_steps)
+        s_per_step, t_init, core = cpu_baseline(model, (pops, conns), delay, args.seed, net, args.cpu_steps)
         cpu = {"value": round(s_per_step / dt, 3), "unit": "wall s / bio s", "cores": 1, "kind": "port",
+               "cpu": host_cpu(),
                "sample": f"{args.cpu_steps} steps of the same {E:.3g}-synapse network (graph copied from the GPU, "
-                         f"state from init; dense per-step plasticity as in cpu::snn), "
-                         f"{s_per_step:.2f} s/step, wall/bio extrapolated linearly; init {t_init:.1f}s excluded"}
+                         f"state from init; dense per-step plasticity as in cpu::snn), one thread pinned to core "
+                         f"{core}, {s_per_step:.2f} s/step, wall/bio extrapolated linearly; init {t_init:.1f}s excluded"}
 
     net_launch = net.launch()
     net.close()
