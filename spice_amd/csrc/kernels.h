@@ -1,22 +1,23 @@
 #pragma once
 // Model-templated step kernels for gfx950 (subsystems b, c, d).
 //
-// Per step (step index T read from device memory, see dview.h):
-//   k_receive<M> : deliver the spikes emitted at T - D. Work items are (spike, part): a
-//                  wave64 walks a contiguous slice of the spike's CSR row (coalesced target
-//                  reads) and calls M::neuron::receive for each target through a proxy
-//                  iterator that routes backend.atomic_add into per-attribute accumulators
-//                  (fp32 atomics, or int64 fixed point). Plastic models first bring each
-//                  plastic synapse up to date (lazy replay, or the deferred events of the
-//                  previous step for neuron-trace synapses) and then receive reads it.
+// Per step T (step index read from device memory, see dview.h), two kernels on one stream:
+//   k_receive_tiled<M> : delivers the spikes emitted at T - D. One workgroup per (target tile,
+//                  part of the step's spike list): the spikes' row segments inside the tile
+//                  are walked as one flat index space and M::neuron::receive accumulates
+//                  into the tile's LDS accumulators (fp32, or int64 fixed point), which then
+//                  go to the global ones. Plastic models bring each delivered plastic synapse
+//                  up to date first (lazy replay, or the previous step's deferred events for
+//                  neuron-trace synapses), and extra workgroups of the same kernel do step
+//                  T - 1's plasticity work (eager pre/post passes, or the lazy flush), which
+//                  touches no row delivered at T.
 //   k_update<M>  : one thread per neuron: add accumulated inputs, run M::neuron::update,
-//                  store, and append spikes to the ring with one atomic per wave
+//                  store, and append spikes to the ring with one atomic per workgroup
 //                  (ballot + popcount + mbcnt). Plastic models also record the spike
-//                  history and queue rows (and, eagerly, spiking targets) for k_flush.
-//   k_flush<M>   : plastic models only: bring queued rows up to date (lazy), or apply this
-//                  step's pre and post events (neuron-trace synapses, eager).
-// The last kernel of a step advances the device clock (last-block-done), so the same
-// launches are valid at every step and can be captured once into a HIP graph.
+//                  history and queue the rows (and, eagerly, spiking targets) for step T's
+//                  plasticity work. Block 0 writes step T + 1's clock and empties its counters,
+//                  so the same launches are valid at every step and are captured once into a
+//                  HIP graph.
 //
 // Reference counterparts: _process_neurons (spice/cuda/algorithm.cu:181-230),
 // _process_spikes HNDL_SPKS / UPDT_SYNS (algorithm.cu:238-313), _gather (algorithm.cu:340-384),

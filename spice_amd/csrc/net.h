@@ -776,7 +776,7 @@ private:
 		_lc = c;
 		// statistics slots: one per workgroup of the widest step kernel (receive: per work item;
 		// eager plasticity: after those, one per plasticity workgroup or synchronization block)
-		// (flush: two sets, the flushes of consecutive steps may overlap; tiled flush: per tile)
+		// (and enough for the update kernel's blocks)
 		std::size_t const slots =
 		    std::size_t( std::max( { _grid_update, 2 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts + std::max( _grid_plast, _grid_flush ) } ) ) * NSTAT;
 		if( slots > _bstats.size() )
