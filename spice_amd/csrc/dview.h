@@ -41,8 +41,9 @@ constexpr int MAXA = 16;
 
 // Device clock. Four slots: the kernels of step T read slot T & 3 and the neuron-update
 // kernel writes step T + 1's clock into slot (T + 1) & 3, so the clock advances without
-// atomics or an extra kernel, the flush of step T (which may run while step T + 1 delivers
-// and updates) still reads its own slot, and a captured graph of a multiple of 4 steps replays.
+// atomics or an extra kernel, the plasticity work of step T (which runs in step T + 1's
+// delivery kernel) still reads its own slot (ctl_prev), and a captured graph of a multiple of
+// 4 steps replays.
 constexpr int CLOCK_SLOTS = 4;
 struct ctl_t
 {
