@@ -103,7 +103,7 @@ public:
 		upload( _flush_local, fl );
 		upload( _flush_count, fc );
 		for( int p = 0; p < CLOCK_SLOTS; p++ ) upload( _views[p], vs[p] );
-		_grid_tiles = std::min( _ntiles, ( recv_block == 256 ? 64 : 16 ) * device_cu_count() );
+		_grid_tiles = std::min( _ntiles, ( recv_block == 256 ? SPICE_BATCH_GRID_PER_CU : SPICE_BATCH_GRID_PER_CU / 4 ) * device_cu_count() );
 		for( auto * n : _nets ) n->synchronize();
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}

@@ -385,11 +385,17 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 }
 
 // ---- target-tiled spike delivery -----------------------------------------------------------
+#ifndef SPICE_UPDATE_BLOCK
+#define SPICE_UPDATE_BLOCK 256 // threads per neuron-update workgroup of a single net (64..256)
+#endif
 #ifndef SPICE_RECV_UNROLL
 #define SPICE_RECV_UNROLL 2
 #endif
 #ifndef SPICE_BATCH_ITEMS_PER_CU
 #define SPICE_BATCH_ITEMS_PER_CU 64
+#endif
+#ifndef SPICE_BATCH_GRID_PER_CU
+#define SPICE_BATCH_GRID_PER_CU 64 // batched delivery workgroups (of 256 threads) per CU, at most
 #endif
 #ifndef SPICE_BATCH_RECV_BLOCK
 #define SPICE_BATCH_RECV_BLOCK 256
@@ -793,7 +799,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 	}
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
-	std::int64_t const i = blk * 256 + threadIdx.x;
+	std::int64_t const i = blk * blockDim.x + threadIdx.x;
 	bool spiked = false, flush = false, post = false;
 	std::int32_t fl_synced = 0, fl_pend = 0;
 
@@ -910,7 +916,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 	if( threadIdx.x == 0 )
 	{
 		unsigned t0 = 0, t1 = 0, t2 = 0;
-		for( int k = 0; k < 4; k++ )
+		for( int k = 0; k < int( blockDim.x >> 6 ); k++ )
 		{
 			unsigned const a = wcount[k][0], b = wcount[k][1], c = wcount[k][2];
 			wcount[k][0] = t0;
@@ -961,7 +967,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 }
 
 template <typename Model, bool FIXED>
-__global__ __launch_bounds__( 256 ) void k_update( dview v )
+__global__ __launch_bounds__( SPICE_UPDATE_BLOCK ) void k_update( dview v )
 {
 	update_block<Model, FIXED>( v, blockIdx.x );
 }

@@ -541,6 +541,7 @@ private:
 	dview _vs[CLOCK_SLOTS]{};
 
 	int _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
+	int _grid_upd = 0; // workgroups of this net's own update kernel (SPICE_UPDATE_BLOCK threads)
 	int _grid_plast = 0; // plasticity workgroups of the delivery kernel
 	launch_cfg _lc{};
 	dbuf<std::int32_t> _pivots;
@@ -668,6 +669,7 @@ private:
 
 		if constexpr( traced ) build_by_target();
 		_grid_update = int( ( n + 255 ) / 256 );
+		_grid_upd = int( ( n + SPICE_UPDATE_BLOCK - 1 ) / SPICE_UPDATE_BLOCK );
 		apply_launch( launch_cfg{} );
 	}
 
@@ -778,7 +780,7 @@ private:
 		// eager plasticity: after those, one per plasticity workgroup or synchronization block)
 		// (and enough for the update kernel's blocks)
 		std::size_t const slots =
-		    std::size_t( std::max( { _grid_update, 2 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts + std::max( _grid_plast, _grid_flush ) } ) ) * NSTAT;
+		    std::size_t( std::max( { _grid_update, _grid_upd, 2 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts + std::max( _grid_plast, _grid_flush ) } ) ) * NSTAT;
 		if( slots > _bstats.size() )
 		{
 			std::vector<unsigned long long> keep( NSTAT, 0ull );
@@ -829,12 +831,12 @@ private:
 		if( fixed() )
 		{
 			launch_receive<true>( v );
-			SPICE_LAUNCH( k_update<Model, true><<<_grid_update, 256, 0, _s>>>( v ) );
+			SPICE_LAUNCH( k_update<Model, true><<<_grid_upd, SPICE_UPDATE_BLOCK, 0, _s>>>( v ) );
 		}
 		else
 		{
 			launch_receive<false>( v );
-			SPICE_LAUNCH( k_update<Model, false><<<_grid_update, 256, 0, _s>>>( v ) );
+			SPICE_LAUNCH( k_update<Model, false><<<_grid_upd, SPICE_UPDATE_BLOCK, 0, _s>>>( v ) );
 		}
 	}
 
@@ -857,7 +859,7 @@ private:
 		SPICE_HIP( hipEventRecord( e[0], _s ) );
 		launch_receive<F>( v );
 		SPICE_HIP( hipEventRecord( e[1], _s ) );
-		SPICE_LAUNCH( k_update<Model, F><<<_grid_update, 256, 0, _s>>>( v ) );
+		SPICE_LAUNCH( k_update<Model, F><<<_grid_upd, SPICE_UPDATE_BLOCK, 0, _s>>>( v ) );
 		SPICE_HIP( hipEventRecord( e[2], _s ) );
 		SPICE_HIP( hipEventRecord( e[3], _s ) );
 	}
