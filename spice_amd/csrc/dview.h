@@ -76,11 +76,13 @@ struct slog_t
 	std::uint64_t cap;
 };
 
-// One plastic synapse of the by-target index: its source row and its offset in that row.
+// One plastic synapse of the by-target index: its source row and its offset in that row
+// (offset first: read as one little-endian 64-bit word the source is the high half, so the
+// word orders entries by source).
 struct alignas( 8 ) tp_entry
 {
-	std::int32_t src;
 	std::int32_t off;
+	std::int32_t src;
 };
 
 struct dview

@@ -1336,8 +1336,8 @@ static inline __global__ __launch_bounds__( 1024 ) void k_tp_scan( std::uint32_t
 }
 
 // Fills the index: every plastic synapse appends (source, offset in the source's row) to its
-// target's list (cursor: a copy of ptr). The order inside a list is arbitrary: each entry is
-// an independent synapse.
+// target's list (cursor: a copy of ptr), in arbitrary order; the host then sorts each list by
+// source.
 template <typename Model>
 __global__ __launch_bounds__( 256 ) void k_tp_fill( dview v, unsigned long long * cursor, tp_entry * ent )
 {
@@ -1352,7 +1352,7 @@ __global__ __launch_bounds__( 256 ) void k_tp_fill( dview v, unsigned long long 
 		if( S::plastic( std::int32_t( r ), j, info ) )
 		{
 			unsigned long long const k = atomicAdd( &cursor[j], 1ull );
-			ent[k] = tp_entry{ std::int32_t( r ), std::int32_t( e - r0 ) };
+			ent[k] = tp_entry{ std::int32_t( e - r0 ), std::int32_t( r ) };
 		}
 	}
 }

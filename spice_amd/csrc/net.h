@@ -13,6 +13,8 @@
 #include <spice/util/numeric.h>
 #include <spice_hip.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <climits>
 #include <cstdint>
@@ -710,6 +712,24 @@ private:
 			dbuf<unsigned long long> cursor( n );
 			SPICE_HIP( hipMemcpyAsync( cursor.data(), _tp_ptr.data(), 8 * n, hipMemcpyDeviceToDevice, _s ) );
 			SPICE_LAUNCH( k_tp_fill<Model><<<rows_grid, 256, 0, _s>>>( _v, cursor.data(), _tp_ent.data() ) );
+			SPICE_HIP( hipStreamSynchronize( _s ) ); // before the temporaries are freed
+		}
+		if( total > 1 && total <= INT_MAX )
+		{
+			// each target's list sorted by source: neighbouring entries of a post-pass wave then
+			// gather neighbouring traces, flags and row starts (fewer cache lines), and the index
+			// does not depend on the order of the fill's atomics. Radix sort on the source bits
+			// of the entries read as 64-bit words.
+			int const bits = 64 - __builtin_clzll( static_cast<unsigned long long>( std::max<std::int64_t>( n - 1, 1 ) ) );
+			auto const * keys = reinterpret_cast<unsigned long long const *>( _tp_ent.data() );
+			dbuf<unsigned long long> sorted{ std::size_t( total ) };
+			std::size_t tb = 0;
+			SPICE_HIP( hipcub::DeviceSegmentedRadixSort::SortKeys( nullptr, tb, keys, sorted.data(), int( total ), int( n ), _tp_ptr.data(),
+			                                                       _tp_ptr.data() + 1, 32, 32 + bits, _s ) );
+			dbuf<char> temp( std::max<std::size_t>( tb, 1 ) );
+			SPICE_HIP( hipcub::DeviceSegmentedRadixSort::SortKeys( temp.data(), tb, keys, sorted.data(), int( total ), int( n ), _tp_ptr.data(),
+			                                                       _tp_ptr.data() + 1, 32, 32 + bits, _s ) );
+			SPICE_HIP( hipMemcpyAsync( _tp_ent.data(), sorted.data(), std::size_t( total ) * 8, hipMemcpyDeviceToDevice, _s ) );
 			SPICE_HIP( hipStreamSynchronize( _s ) ); // before the temporaries are freed
 		}
 		// targets with incoming plastic synapses (the mean in-degree sets the post-pass split)

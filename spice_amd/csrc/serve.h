@@ -66,8 +66,8 @@ inline std::uint64_t estimate_job_bytes( int model, std::int64_t n, std::int64_t
 		{
 			b += 8 * HWORDS * N + 36 * N;
 			// eager: traces, flags and target lists by parity, by-target index (at most every
-			// edge) with its offsets and build temporaries
-			if constexpr( has_neuron_traces_v<Ss> ) b += 8 * E + 48 * N + 64;
+			// edge) with its offsets and build temporaries (the sort's second buffer and work space)
+			if constexpr( has_neuron_traces_v<Ss> ) b += 17 * E + 48 * N + ( 64 << 20 );
 		}
 		b += 8 * NSTAT * ( N / 256 + 8 * cus + ntiles + 1 );             // statistics slots
 		b += 4 * spike_log_capacity( n, chunk, dt ) + 8 * std::uint64_t( chunk ) + 64;
