@@ -1,5 +1,5 @@
-"""Statistics over 10 s of biological time: per-population firing rates and ISI coefficients
-of variation of the GPU (fp32 accumulation) within 2 % of the CPU checker.
+"""Statistics over 10 s of biological time (2 s with STDP): per-population firing rates and ISI
+coefficients of variation of the GPU (fp32 accumulation) within 2 % of the CPU checker.
 
 fp32 sums are ordered differently on the two sides, so the trajectories diverge after a few
 hundred steps and only statistics can agree. For the Brunel network at 1e6 synapses (3,162
@@ -25,16 +25,16 @@ STEPS = 100_000  # 10 s
 TOL = 0.02
 
 
-def _cpu_stats(model, lay, delay, seed):
+def _cpu_stats(model, lay, delay, seed, steps):
     h = O.CpuSnn(model, lay, DT, delay, seed=seed)
-    off, ids = h.run_record(STEPS)
+    off, ids = h.run_record(steps)
     h.close()
     r = Raster(sum(lay[0]), lay[0])
     r.add(0, off, ids)
     return r.rates(DT), r.isi_cv()
 
 
-def _gpu_stats(model, lay, delay, seeds):
+def _gpu_stats(model, lay, delay, seeds, steps):
     pops = lay[0]
     rasters, lock = {}, threading.Lock()
 
@@ -44,20 +44,22 @@ def _gpu_stats(model, lay, delay, seeds):
         r.add(first, off, ids)
 
     with Server(budget_bytes=16e9, chunk_steps=2000, on_spikes=sink) as sv:
-        ids = [sv.submit(model, Layout(*lay), DT, STEPS, delay=delay, seed=s) for s in seeds]
+        ids = [sv.submit(model, Layout(*lay), DT, steps, delay=delay, seed=s) for s in seeds]
         recs = [sv.wait(i, timeout=600) for i in ids]
     assert all(r["status"] == "done" for r in recs), recs
     return [(rasters[i].rates(DT), rasters[i].isi_cv()) for i in ids]
 
 
-@pytest.mark.parametrize("model,lay,delay,seeds", [
-    ("brunel_3pop", three_pop_layout(3162), 15, range(1, 17)),
-    ("vogels_abbott", ([10000], [(0, 0, 0.02)]), 1, range(1, 5)),
+@pytest.mark.parametrize("model,lay,delay,seeds,steps", [
+    ("brunel_3pop", three_pop_layout(3162), 15, range(1, 17), STEPS),
+    ("vogels_abbott", ([10000], [(0, 0, 0.02)]), 1, range(1, 5), STEPS),
+    # fp32 STDP (eager on the GPU, dense per step in the checker): 2 s, weights move throughout
+    ("brunel_3pop_stdp", three_pop_layout(3162), 15, range(1, 9), 20_000),
 ])
-def test_rates_and_isi_cv_match_checker(gpu, model, lay, delay, seeds):
+def test_rates_and_isi_cv_match_checker(gpu, model, lay, delay, seeds, steps):
     with ThreadPoolExecutor(8) as ex:  # the checker releases the GIL while it runs
-        cpu = list(ex.map(lambda s: _cpu_stats(model, lay, delay, s), seeds))
-    gpu_ = _gpu_stats(model, lay, delay, seeds)
+        cpu = list(ex.map(lambda s: _cpu_stats(model, lay, delay, s, steps), seeds))
+    gpu_ = _gpu_stats(model, lay, delay, seeds, steps)
     c_rate, c_cv = np.mean([c[0] for c in cpu], 0), np.mean([c[1] for c in cpu], 0)
     g_rate, g_cv = np.mean([g[0] for g in gpu_], 0), np.mean([g[1] for g in gpu_], 0)
     print(f"{model}: rates cpu {c_rate} gpu {g_rate}; ISI CV cpu {c_cv} gpu {g_cv}")
