@@ -719,7 +719,8 @@ private:
 			// each target's list sorted by source: neighbouring entries of a post-pass wave then
 			// gather neighbouring traces, flags and row starts (fewer cache lines), and the index
 			// does not depend on the order of the fill's atomics. Radix sort on the source bits
-			// of the entries read as 64-bit words.
+			// of the entries read as 64-bit words (the sort counts items in int: an index of more
+			// than 2^31 entries keeps the fill's order, which is equally correct).
 			int const bits = 64 - __builtin_clzll( static_cast<unsigned long long>( std::max<std::int64_t>( n - 1, 1 ) ) );
 			auto const * keys = reinterpret_cast<unsigned long long const *>( _tp_ent.data() );
 			dbuf<unsigned long long> sorted{ std::size_t( total ) };
