@@ -388,9 +388,6 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 #ifndef SPICE_UPDATE_BLOCK
 #define SPICE_UPDATE_BLOCK 256 // threads per neuron-update workgroup of a single net (64..256)
 #endif
-#ifndef SPICE_RECV_UNROLL
-#define SPICE_RECV_UNROLL 2
-#endif
 #ifndef SPICE_BATCH_ITEMS_PER_CU
 #define SPICE_BATCH_ITEMS_PER_CU 64
 #endif
@@ -492,28 +489,40 @@ __device__ inline std::int32_t block_exclusive_scan( std::int32_t x, std::int32_
 	return wb + std::int32_t( incl ) - x;
 }
 
-// Largest k in [0, cnt) with pre[k] <= f (pre non-decreasing, pre[0] = 0).
-__device__ inline std::int32_t segment_of( std::int32_t const * pre, std::int32_t cnt, std::int32_t f )
+// For each u: the largest k[u] in [0, cnt) with pre[k[u]] <= f[u] (pre non-decreasing,
+// pre[0] = 0 <= f[u], cnt >= 1). Binary lifting with step sizes from the largest power of two
+// below cnt down to 1, branch-free: the U searches advance together, so each step's U LDS
+// reads are in flight at once (log2(cnt) dependent reads in all, not U times that).
+template <int U>
+__device__ inline void segments_of( std::int32_t const * pre, std::int32_t cnt, std::int32_t const ( &f )[U], std::int32_t ( &k )[U] )
 {
-	std::int32_t a = 0, b = cnt - 1;
-	while( a < b )
+#pragma unroll
+	for( int u = 0; u < U; u++ ) k[u] = 0;
+	for( std::int32_t st = cnt > 1 ? std::int32_t( 1u << ( 31 - __builtin_clz( std::uint32_t( cnt - 1 ) ) ) ) : 0; st > 0; st >>= 1 )
 	{
-		std::int32_t const m = ( a + b + 1 ) >> 1;
-		if( pre[m] <= f )
-			a = m;
-		else
-			b = m - 1;
+		std::int32_t p[U];
+#pragma unroll
+		for( int u = 0; u < U; u++ )
+		{
+			std::int32_t const m = k[u] + st;
+			p[u] = pre[m < cnt ? m : cnt - 1];
+		}
+#pragma unroll
+		for( int u = 0; u < U; u++ )
+		{
+			std::int32_t const m = k[u] + st;
+			k[u] = ( m < cnt && p[u] <= f[u] ) ? m : k[u];
+		}
 	}
-	return a;
 }
 
 // Delivers the spikes of step T - D, one workgroup per tile of tile_w target neurons:
 // the workgroup stages a chunk of spikes (id, segment of the row inside the tile, and for
 // plastic models the row's lazy state) in LDS, forms the segments' prefix sums, and walks
-// the concatenated segments as one flat index space (4 entries per thread in flight, the
-// owning spike found by binary search in LDS). receive() accumulates into LDS with LDS
-// atomics; at the end the tile's sums go to the global accumulators with plain stores:
-// one workgroup owns each tile, so no global atomics are needed.
+// the concatenated segments as one flat index space (2 or 4 entries per thread in flight,
+// the owning spikes found by interleaved binary searches in LDS). receive() accumulates into
+// LDS with LDS atomics; at the end the tile's sums go to the global accumulators with plain
+// stores: one workgroup owns each tile, so no global atomics are needed.
 template <typename Model, bool FIXED, int BS>
 __device__ inline void receive_tile( dview const & v, std::int32_t work )
 {
@@ -526,7 +535,6 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	constexpr bool plastic = is_plastic_v<Model>;
 	constexpr std::uint32_t rmask = receive_mask_v<N>;
 	constexpr int nacc = __builtin_popcount( rmask ) > 0 ? __builtin_popcount( rmask ) : 1;
-	constexpr int U = SPICE_RECV_UNROLL; // delivered events per thread in flight
 
 	__shared__ unsigned long long acc[nacc * TILE_MAX];
 	__shared__ std::int64_t seg_beg[BS];
@@ -595,20 +603,25 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 			__syncthreads();
 			std::int32_t const total = seg_pre[BS];
 
-			for( std::int32_t f0 = 0; f0 < total; f0 += BS * U )
-			{
-				std::int32_t kk[U];
+			// the flat walk: 4 events per thread in flight while more than two workgroups' worth
+			// remain, then 2 (a short last pass, or a small tile, runs fewer idle searches)
+			auto walk = [&]( auto uu, std::int32_t f0 ) {
+				constexpr int U = decltype( uu )::value;
+				std::int32_t kk[U], ff[U];
 				std::int64_t e[U];
 				std::int32_t j[U];
 				bool ok[U];
 #pragma unroll
 				for( int u = 0; u < U; u++ )
 				{
-					std::int32_t const f = f0 + tid + BS * u;
-					ok[u] = f < total;
-					std::int32_t const a = segment_of( seg_pre, cnt, f );
-					kk[u] = a;
-					e[u] = seg_beg[a] + ( f - seg_pre[a] );
+					ff[u] = f0 + tid + BS * u;
+					ok[u] = ff[u] < total;
+				}
+				segments_of<U>( seg_pre, cnt, ff, kk );
+#pragma unroll
+				for( int u = 0; u < U; u++ )
+				{
+					e[u] = seg_beg[kk[u]] + ( ff[u] - seg_pre[kk[u]] );
 					j[u] = ok[u] ? lds<( SPICE_NT & 4 ) != 0>( v.col + e[u] ) : 0;
 				}
 				if constexpr( eager )
@@ -678,7 +691,10 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 						N::receive( s, dst, no_syn_iter{}, info, bak );
 				}
 				}
-			}
+			};
+			std::int32_t f0 = 0;
+			for( ; total - f0 > 2 * BS; f0 += 4 * BS ) walk( std::integral_constant<int, 4>{}, f0 );
+			for( ; f0 < total; f0 += 2 * BS ) walk( std::integral_constant<int, 2>{}, f0 );
 			if( tid == 0 ) ev += std::uint64_t( total );
 			__syncthreads(); // segments are overwritten by the next chunk
 		}
