@@ -18,6 +18,7 @@ struct brunel : model
 			Twait
 		};
 		static constexpr std::uint32_t receive_mask = 1u << V;
+		static constexpr std::uint32_t receive_reads = 1u << Twait;
 
 		template <typename Iter, typename Backend>
 		HYBRID static void init( Iter n, snn_info, Backend & )

@@ -31,6 +31,7 @@ struct brunel_3pop : model
 			Twait
 		};
 		static constexpr std::uint32_t receive_mask = 1u << V;
+		static constexpr std::uint32_t receive_reads = 1u << Twait;
 
 		template <typename Iter, typename Backend>
 		HYBRID static void init( Iter n, snn_info, Backend & )

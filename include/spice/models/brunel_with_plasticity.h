@@ -27,6 +27,7 @@ struct brunel_with_plasticity : model
 	struct neuron : ::spice::neuron<float, int_>
 	{
 		static constexpr std::uint32_t receive_mask = 1u << V;
+		static constexpr std::uint32_t receive_reads = 1u << Twait;
 
 		template <typename Iter, typename Backend>
 		HYBRID static void init( Iter n, snn_info, Backend & )

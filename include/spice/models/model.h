@@ -24,10 +24,14 @@
 //  * synapse::update(n, pre, post) must equal n-1 event-free steps followed by one step
 //    with the given flags (this is what lets plasticity be applied lazily).
 //
-// Optional trait: a neuron type may declare
+// Optional traits: a neuron type may declare
 //     static constexpr std::uint32_t receive_mask = (1u << A) | (1u << B);
-// naming the attributes receive() adds to. Without it every attribute gets an input
-// accumulator.
+// naming the attributes receive() adds to (without it every attribute gets an input
+// accumulator), and
+//     static constexpr std::uint32_t receive_reads = 1u << C;
+// naming the attributes receive() reads from its target (without it, none are assumed: reads
+// then go to device memory for every delivered event; with it, the delivering workgroup keeps
+// a copy of its tile's values of those attributes in LDS). Neither changes results.
 #include <spice/snn_info.h>
 #include <spice/util/host_defines.h>
 #include <spice/util/meta.h>
@@ -142,6 +146,24 @@ struct receive_mask_of<N, std::void_t<decltype( N::receive_mask )>>
 // Bit I set <=> neuron attribute I can be the target of receive()'s atomic_add.
 template <typename Neuron>
 inline constexpr std::uint32_t receive_mask_v = detail::receive_mask_of<Neuron>::value;
+
+namespace detail
+{
+template <typename N, typename = void>
+struct receive_reads_of
+{
+	static constexpr std::uint32_t value = 0;
+};
+template <typename N>
+struct receive_reads_of<N, std::void_t<decltype( N::receive_reads )>>
+{
+	static constexpr std::uint32_t value = N::receive_reads;
+};
+} // namespace detail
+
+// Bit I set <=> receive() reads neuron attribute I of its target.
+template <typename Neuron>
+inline constexpr std::uint32_t receive_reads_v = detail::receive_reads_of<Neuron>::value;
 
 template <typename Model>
 inline constexpr bool is_plastic_v = Model::synapse::size > 0;

@@ -32,6 +32,7 @@
 #include <cstdint>
 #include <tuple>
 #include <type_traits>
+#include <utility>
 
 namespace spice::hipx
 {
@@ -426,16 +427,39 @@ static inline __global__ __launch_bounds__( 256 ) void k_pivots( dview v )
 	}
 }
 
-// Receive target inside a tile: reads see the neuron state, backend.atomic_add goes to the
-// tile's LDS accumulator of attribute I.
-template <typename T, int I, typename A, bool InMask>
+// Attributes of the receiving neurons that the delivering workgroup copies into LDS: those
+// that receive() declares it reads (receive_reads), of at most 4 bytes.
+template <typename Neuron, std::size_t... I>
+constexpr std::uint32_t staged_mask_of( std::index_sequence<I...> )
+{
+	return ( 0u | ... | ( ( ( receive_reads_v<Neuron> >> I ) & 1u ) != 0 && sizeof( typename Neuron::template type<int( I )> ) <= 4 ? 1u << I : 0u ) );
+}
+template <typename Neuron>
+constexpr std::uint32_t staged_mask()
+{
+	return staged_mask_of<Neuron>( std::make_index_sequence<std::size_t( Neuron::size )>{} );
+}
+
+// Receive target inside a tile: reads see the neuron state (from the tile's LDS copy when the
+// attribute is staged), backend.atomic_add goes to the tile's LDS accumulator of attribute I.
+template <typename T, int I, typename A, bool InMask, bool Staged = false>
 struct tile_proxy
 {
 	dview const * v;
 	std::int32_t j;
 	A * lds; // accumulator of attribute I for this tile, indexed by j - tile start
 	std::int32_t t0;
-	__device__ operator T() const { return ld<T>( v->na[I], j ); }
+	std::uint32_t const * stage; // Staged: attribute I's values of the tile, by j - tile start
+	__device__ operator T() const
+	{
+		if constexpr( Staged )
+		{
+			using B = std::conditional_t<sizeof( T ) == 1, std::uint8_t, std::conditional_t<sizeof( T ) == 2, std::uint16_t, std::uint32_t>>;
+			return __builtin_bit_cast( T, B( stage[j - t0] ) );
+		}
+		else
+			return ld<T>( v->na[I], j );
+	}
 };
 
 template <typename Neuron, bool FIXED>
@@ -444,15 +468,19 @@ struct tile_iter
 	dview const * v;
 	std::int32_t j;
 	std::int32_t t0;
-	unsigned long long * lds; // [nacc][TILE_MAX] 8-byte slots
+	unsigned long long * lds;    // [nacc][TILE_MAX] 8-byte slots
+	std::uint32_t const * stage; // [nstaged][TILE_MAX] 4-byte slots
 	template <int I>
 	__device__ auto get() const
 	{
 		using T = typename Neuron::template type<I>;
 		constexpr std::uint32_t mask = receive_mask_v<Neuron>;
+		constexpr std::uint32_t smask = staged_mask<Neuron>();
 		constexpr int a = __builtin_popcount( mask & ( ( 1u << I ) - 1u ) );
+		constexpr int b = __builtin_popcount( smask & ( ( 1u << I ) - 1u ) );
 		using A = std::conditional_t<std::is_floating_point_v<T>, std::conditional_t<FIXED, long long, float>, T>;
-		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0>{ v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0 };
+		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, ( ( smask >> I ) & 1u ) != 0>{
+		    v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0, stage + b * TILE_MAX };
 	}
 	__device__ uint_ id() const { return uint_( j ); }
 };
@@ -461,8 +489,8 @@ template <bool FIXED>
 struct tile_backend : dev_backend<FIXED>
 {
 	using dev_backend<FIXED>::atomic_add;
-	template <typename T, int I, typename A, bool InMask>
-	__device__ void atomic_add( tile_proxy<T, I, A, InMask> p, typename util::identity<T>::type x ) const
+	template <typename T, int I, typename A, bool InMask, bool Staged>
+	__device__ void atomic_add( tile_proxy<T, I, A, InMask, Staged> p, typename util::identity<T>::type x ) const
 	{
 		static_assert( InMask, "receive() adds to a neuron attribute that the model's receive_mask does not list" );
 		A * dst = p.lds + ( p.j - p.t0 );
@@ -536,7 +564,10 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	constexpr std::uint32_t rmask = receive_mask_v<N>;
 	constexpr int nacc = __builtin_popcount( rmask ) > 0 ? __builtin_popcount( rmask ) : 1;
 
+	constexpr std::uint32_t smask = staged_mask<N>();
+	constexpr int nstaged = __builtin_popcount( smask );
 	__shared__ unsigned long long acc[nacc * TILE_MAX];
+	__shared__ std::uint32_t stage[nstaged > 0 ? nstaged * TILE_MAX : 1];
 	__shared__ std::int64_t seg_beg[BS];
 	__shared__ std::int32_t seg_pre[BS + 1];
 	__shared__ std::int32_t seg_src[BS];
@@ -572,6 +603,17 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
 		for( int x = tid; x < nacc * TILE_MAX; x += BS ) acc[x] = 0ull;
+		// the tile's values of the attributes receive() reads (they do not change during
+		// delivery: the neuron update runs after it)
+		util::for_n<N::size>( [&]( auto I ) {
+			if constexpr( ( ( smask >> I ) & 1u ) != 0 )
+			{
+				using T = typename N::template type<I>;
+				using B = std::conditional_t<sizeof( T ) == 1, std::uint8_t, std::conditional_t<sizeof( T ) == 2, std::uint16_t, std::uint32_t>>;
+				constexpr int b = __builtin_popcount( smask & ( ( 1u << I ) - 1u ) );
+				for( int x = tid; x < tw; x += BS ) stage[b * TILE_MAX + x] = std::uint32_t( __builtin_bit_cast( B, ld<T>( v.na[I], t0 + x ) ) );
+			}
+		} );
 		__syncthreads();
 
 		for( std::int32_t c0 = 0; c0 < nspk; c0 += BS )
@@ -649,7 +691,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 						if( !ok[u] ) continue;
 						std::int32_t const s = seg_src[kk[u]];
 						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
-						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc };
+						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc, stage };
 						reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
 						bool const pre = ( f8[u] & 0x80 ) != 0, post = ( f8[u] & F8_SPIKED ) != 0;
 						if( pre || post )
@@ -672,7 +714,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					if( !ok[u] ) continue;
 					std::int32_t const s = seg_src[kk[u]];
 					tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
-					tile_iter<N, FIXED> dst{ &v, j[u], t0, acc };
+					tile_iter<N, FIXED> dst{ &v, j[u], t0, acc, stage };
 					if constexpr( plastic )
 					{
 						typename S::tuple_t sy;
