@@ -462,14 +462,28 @@ struct tile_proxy
 	}
 };
 
+// Accumulator type of a received attribute of type T: fp32, or int64 fixed point, for floating
+// point attributes; T itself otherwise.
+template <typename T, bool FIXED>
+using acc_type = std::conditional_t<std::is_floating_point_v<T>, std::conditional_t<FIXED, long long, float>, T>;
+
+// LDS word of the tile accumulators: 4 bytes unless an accumulator needs 8.
+template <typename Neuron, bool FIXED, std::size_t... I>
+constexpr bool acc_needs_8( std::index_sequence<I...> )
+{
+	return ( false || ... || ( ( ( receive_mask_v<Neuron> >> I ) & 1u ) != 0 && sizeof( acc_type<typename Neuron::template type<int( I )>, FIXED> ) > 4 ) );
+}
+template <typename Neuron, bool FIXED>
+using acc_word = std::conditional_t<acc_needs_8<Neuron, FIXED>( std::make_index_sequence<std::size_t( Neuron::size )>{} ), unsigned long long, std::uint32_t>;
+
 template <typename Neuron, bool FIXED>
 struct tile_iter
 {
 	dview const * v;
 	std::int32_t j;
 	std::int32_t t0;
-	unsigned long long * lds;    // [nacc][TILE_MAX] 8-byte slots
-	std::uint32_t const * stage; // [nstaged][TILE_MAX] 4-byte slots
+	acc_word<Neuron, FIXED> * lds; // [nacc][TILE_MAX] accumulator slots
+	std::uint32_t const * stage;   // [nstaged][TILE_MAX] 4-byte slots
 	template <int I>
 	__device__ auto get() const
 	{
@@ -478,7 +492,7 @@ struct tile_iter
 		constexpr std::uint32_t smask = staged_mask<Neuron>();
 		constexpr int a = __builtin_popcount( mask & ( ( 1u << I ) - 1u ) );
 		constexpr int b = __builtin_popcount( smask & ( ( 1u << I ) - 1u ) );
-		using A = std::conditional_t<std::is_floating_point_v<T>, std::conditional_t<FIXED, long long, float>, T>;
+		using A = acc_type<T, FIXED>;
 		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, ( ( smask >> I ) & 1u ) != 0>{
 		    v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0, stage + b * TILE_MAX };
 	}
@@ -502,19 +516,20 @@ struct tile_backend : dev_backend<FIXED>
 	}
 };
 
-// Exclusive scan of one int32 per thread over the block (a multiple of 64 threads): returns
-// this thread's exclusive prefix and writes the block total to *total (LDS). Contains barriers.
-__device__ inline std::int32_t block_exclusive_scan( std::int32_t x, std::int32_t * wsum, std::int32_t * total )
+// Exclusive scan of one 64-bit value per thread over the block (a multiple of 64 threads):
+// returns this thread's exclusive prefix and writes the block total to *total (LDS). Contains
+// barriers.
+__device__ inline std::uint64_t block_exclusive_scan( std::uint64_t x, std::uint64_t * wsum, std::uint64_t * total )
 {
-	std::uint64_t const incl = wave_inclusive_scan( std::uint64_t( std::uint32_t( x ) ) );
+	std::uint64_t const incl = wave_inclusive_scan( x );
 	int const tid = threadIdx.x;
-	if( lane_id() == 63 ) wsum[tid >> 6] = std::int32_t( incl );
+	if( lane_id() == 63 ) wsum[tid >> 6] = incl;
 	__syncthreads();
-	std::int32_t wb = 0;
+	std::uint64_t wb = 0;
 	for( int w = 0; w < ( tid >> 6 ); w++ ) wb += wsum[w];
-	if( tid == int( blockDim.x ) - 1 ) *total = wb + std::int32_t( incl );
+	if( tid == int( blockDim.x ) - 1 ) *total = wb + incl;
 	__syncthreads();
-	return wb + std::int32_t( incl ) - x;
+	return wb + incl - x;
 }
 
 // For each u: the largest k[u] in [0, cnt) with pre[k[u]] <= f[u] (pre non-decreasing,
@@ -547,8 +562,9 @@ __device__ inline void segments_of( std::int32_t const * pre, std::int32_t cnt, 
 // Delivers the spikes of step T - D, one workgroup per tile of tile_w target neurons:
 // the workgroup stages a chunk of spikes (id, segment of the row inside the tile, and for
 // plastic models the row's lazy state) in LDS, forms the segments' prefix sums, and walks
-// the concatenated segments as one flat index space (2 or 4 entries per thread in flight,
-// the owning spikes found by interleaved binary searches in LDS). receive() accumulates into
+// the concatenated segments as one flat index space (2 or 4 entries per thread in flight; an
+// entry's segment found from a bitmap of segment starts, or for very large chunks by binary
+// search). receive() accumulates into
 // LDS with LDS atomics; at the end the tile's sums go to the global accumulators with plain
 // stores: one workgroup owns each tile, so no global atomics are needed.
 template <typename Model, bool FIXED, int BS>
@@ -566,16 +582,25 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 
 	constexpr std::uint32_t smask = staged_mask<N>();
 	constexpr int nstaged = __builtin_popcount( smask );
-	__shared__ unsigned long long acc[nacc * TILE_MAX];
+	using word = acc_word<N, FIXED>;
+	__shared__ word acc[nacc * TILE_MAX];
 	__shared__ std::uint32_t stage[nstaged > 0 ? nstaged * TILE_MAX : 1];
+	// a chunk's non-empty segments (spike rows inside the tile), compacted: start in the row
+	// store, first flat event index, source, and for finding an event's segment without a search
+	// (chunks of at most 32 * SW events) a bitmap of the flat indices where segments start and
+	// the segment holding the first event of every 32
+	constexpr int SW = BS;
 	__shared__ std::int64_t seg_beg[BS];
-	__shared__ std::int32_t seg_pre[BS + 1];
+	__shared__ std::int32_t seg_pre[BS];
 	__shared__ std::int32_t seg_src[BS];
+	__shared__ std::uint32_t seg_bits[SW];
+	__shared__ std::uint16_t seg_word[SW];
+	__shared__ std::uint64_t seg_total;
 	constexpr bool eager = plastic && has_neuron_traces_v<S>;
 	__shared__ std::int32_t seg_synced[plastic && !eager ? BS : 1];
 	__shared__ std::int32_t seg_pend[plastic ? BS : 1]; // eager: the row's spike also arrived at T - 1
 	__shared__ float seg_xf[eager ? BS : 1];            // eager: the row's pre trace after T - 1
-	__shared__ std::int32_t wsum[BS / 64];
+	__shared__ std::uint64_t wsum[BS / 64];
 
 	std::int64_t const T = v.ctl->step;
 	int const D1 = v.D + 1;
@@ -602,7 +627,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	{
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
-		for( int x = tid; x < nacc * TILE_MAX; x += BS ) acc[x] = 0ull;
+		for( int x = tid; x < nacc * TILE_MAX; x += BS ) acc[x] = word( 0 );
 		// the tile's values of the attributes receive() reads (they do not change during
 		// delivery: the neuron update runs after it)
 		util::for_n<N::size>( [&]( auto I ) {
@@ -619,31 +644,56 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 		for( std::int32_t c0 = 0; c0 < nspk; c0 += BS )
 		{
 			std::int32_t const cnt = ( nspk - c0 ) < BS ? ( nspk - c0 ) : BS;
-			std::int32_t len = 0;
+			std::int32_t len = 0, src = 0, pend = 0, synced = 0;
+			std::int64_t beg = 0;
+			float xf = 0.0f;
 			if( tid < cnt )
 			{
-				std::int32_t const s = spk[c0 + tid];
-				std::int32_t const * pv = v.pivots + std::int64_t( s ) * ( v.ntiles + 1 ) + tile;
+				src = spk[c0 + tid];
+				std::int32_t const * pv = v.pivots + std::int64_t( src ) * ( v.ntiles + 1 ) + tile;
 				std::int32_t const a = pv[0], b = pv[1];
-				seg_beg[tid] = v.row_ptr[s] + a;
-				seg_src[tid] = s;
+				beg = v.row_ptr[src] + a;
 				len = b - a;
 				if constexpr( eager )
 				{
-					seg_pend[tid] = catch_up && ( v.fl8[prev + s] & F8_ARRIVED );
-					seg_xf[tid] = catch_up ? v.xtr[prev + s] : 0.0f;
+					pend = catch_up && ( v.fl8[prev + src] & F8_ARRIVED );
+					xf = catch_up ? v.xtr[prev + src] : 0.0f;
 				}
 				else if constexpr( plastic )
 				{
-					seg_synced[tid] = v.synced[s];
-					seg_pend[tid] = v.pend[s];
+					synced = v.synced[src];
+					pend = v.pend[src];
 				}
 			}
-			// exclusive block scan of the segment lengths
-			std::int32_t const pre = block_exclusive_scan( len, wsum, &seg_pre[BS] );
-			if( tid < cnt ) seg_pre[tid] = pre;
+			for( int w = tid; w < SW; w += BS ) seg_bits[w] = 0u; // ordered by the scan's barriers
+			// one exclusive block scan of (non-empty segments, events): segment c of the chunk
+			// holds the flat events [pre, pre + len)
+			std::uint64_t const sc = block_exclusive_scan( ( std::uint64_t( len > 0 ) << 32 ) | std::uint32_t( len ), wsum, &seg_total );
+			std::int32_t const pre = std::int32_t( std::uint32_t( sc ) ), c = std::int32_t( sc >> 32 );
+			std::int32_t const total = std::int32_t( std::uint32_t( seg_total ) ), nseg = std::int32_t( seg_total >> 32 );
+			bool const bitmap = total <= 32 * SW;
+			if( len > 0 )
+			{
+				seg_beg[c] = beg;
+				seg_pre[c] = pre;
+				seg_src[c] = src;
+				if constexpr( eager )
+				{
+					seg_pend[c] = pend;
+					seg_xf[c] = xf;
+				}
+				else if constexpr( plastic )
+				{
+					seg_synced[c] = synced;
+					seg_pend[c] = pend;
+				}
+				if( bitmap )
+				{
+					atomicOr( &seg_bits[pre >> 5], 1u << ( pre & 31 ) );
+					for( std::int32_t w = ( pre + 31 ) >> 5; ( w << 5 ) < pre + len; w++ ) seg_word[w] = std::uint16_t( c );
+				}
+			}
 			__syncthreads();
-			std::int32_t const total = seg_pre[BS];
 
 			// the flat walk: 4 events per thread in flight while more than two workgroups' worth
 			// remain, then 2 (a short last pass, or a small tile, runs fewer idle searches)
@@ -659,7 +709,20 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					ff[u] = f0 + tid + BS * u;
 					ok[u] = ff[u] < total;
 				}
-				segments_of<U>( seg_pre, cnt, ff, kk );
+				if( bitmap )
+				{
+					// the segment holding the word's first event, plus the segments that start in
+					// the word after it and at or before the event
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						std::int32_t const f = ok[u] ? ff[u] : total - 1;
+						std::uint32_t const upto = ( 2u << ( f & 31 ) ) - 1u;
+						kk[u] = std::int32_t( seg_word[f >> 5] ) + __builtin_popcount( seg_bits[f >> 5] & upto & ~1u );
+					}
+				}
+				else
+					segments_of<U>( seg_pre, nseg, ff, kk );
 #pragma unroll
 				for( int u = 0; u < U; u++ )
 				{
@@ -748,8 +811,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 		util::for_n<N::size>( [&]( auto I ) {
 			if constexpr( ( ( rmask >> I ) & 1u ) != 0 )
 			{
-				using T = typename N::template type<I>;
-				using A = std::conditional_t<std::is_floating_point_v<T>, std::conditional_t<FIXED, long long, float>, T>;
+				using A = acc_type<typename N::template type<I>, FIXED>;
 				constexpr int a = __builtin_popcount( rmask & ( ( 1u << I ) - 1u ) );
 				A const * src = reinterpret_cast<A const *>( acc + a * TILE_MAX );
 				for( int x = tid; x < tw; x += BS )
