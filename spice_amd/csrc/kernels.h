@@ -585,16 +585,20 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	using word = acc_word<N, FIXED>;
 	__shared__ word acc[nacc * TILE_MAX];
 	__shared__ std::uint32_t stage[nstaged > 0 ? nstaged * TILE_MAX : 1];
-	// a chunk's non-empty segments (spike rows inside the tile), compacted: start in the row
-	// store, first flat event index, source, and for finding an event's segment without a search
-	// (chunks of at most 32 * SW events) a bitmap of the flat indices where segments start and
-	// the segment holding the first event of every 32
+	// a chunk's non-empty segments (spike rows inside the tile), compacted: one record (row-store
+	// offset, source) read per event, the first flat event index, and for finding an event's
+	// segment without a search (chunks of at most 32 * SW events) a bitmap of the flat indices
+	// where segments start, each word paired with the segment holding its first event
 	constexpr int SW = BS;
-	__shared__ std::int64_t seg_beg[BS];
+	struct alignas( 16 ) seg_rec
+	{
+		std::int64_t off; // row-store index of the segment's entry f is off + f
+		std::int32_t src;
+		std::int32_t pad;
+	};
+	__shared__ seg_rec seg[BS];
 	__shared__ std::int32_t seg_pre[BS];
-	__shared__ std::int32_t seg_src[BS];
-	__shared__ std::uint32_t seg_bits[SW];
-	__shared__ std::uint16_t seg_word[SW];
+	__shared__ std::uint64_t seg_map[SW]; // low half: start bitmap word, high half: its first event's segment
 	__shared__ std::uint64_t seg_total;
 	constexpr bool eager = plastic && has_neuron_traces_v<S>;
 	__shared__ std::int32_t seg_synced[plastic && !eager ? BS : 1];
@@ -665,7 +669,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					pend = v.pend[src];
 				}
 			}
-			for( int w = tid; w < SW; w += BS ) seg_bits[w] = 0u; // ordered by the scan's barriers
+			for( int w = tid; w < SW; w += BS ) seg_map[w] = 0ull; // ordered by the scan's barriers
 			// one exclusive block scan of (non-empty segments, events): segment c of the chunk
 			// holds the flat events [pre, pre + len)
 			std::uint64_t const sc = block_exclusive_scan( ( std::uint64_t( len > 0 ) << 32 ) | std::uint32_t( len ), wsum, &seg_total );
@@ -674,9 +678,8 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 			bool const bitmap = total <= 32 * SW;
 			if( len > 0 )
 			{
-				seg_beg[c] = beg;
+				seg[c] = seg_rec{ beg - pre, src, 0 };
 				seg_pre[c] = pre;
-				seg_src[c] = src;
 				if constexpr( eager )
 				{
 					seg_pend[c] = pend;
@@ -689,8 +692,9 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 				}
 				if( bitmap )
 				{
-					atomicOr( &seg_bits[pre >> 5], 1u << ( pre & 31 ) );
-					for( std::int32_t w = ( pre + 31 ) >> 5; ( w << 5 ) < pre + len; w++ ) seg_word[w] = std::uint16_t( c );
+					auto * const words = reinterpret_cast<std::uint32_t *>( seg_map ); // little endian: [2w] bits, [2w + 1] segment
+					atomicOr( &words[2 * ( pre >> 5 )], 1u << ( pre & 31 ) );
+					for( std::int32_t w = ( pre + 31 ) >> 5; ( w << 5 ) < pre + len; w++ ) words[2 * w + 1] = std::uint32_t( c );
 				}
 			}
 			__syncthreads();
@@ -699,7 +703,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 			// remain, then 2 (a short last pass, or a small tile, runs fewer idle searches)
 			auto walk = [&]( auto uu, std::int32_t f0 ) {
 				constexpr int U = decltype( uu )::value;
-				std::int32_t kk[U], ff[U];
+				std::int32_t kk[U], ff[U], sv[U];
 				std::int64_t e[U];
 				std::int32_t j[U];
 				bool ok[U];
@@ -718,7 +722,8 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					{
 						std::int32_t const f = ok[u] ? ff[u] : total - 1;
 						std::uint32_t const upto = ( 2u << ( f & 31 ) ) - 1u;
-						kk[u] = std::int32_t( seg_word[f >> 5] ) + __builtin_popcount( seg_bits[f >> 5] & upto & ~1u );
+						std::uint64_t const m = seg_map[f >> 5];
+						kk[u] = std::int32_t( m >> 32 ) + __builtin_popcount( std::uint32_t( m ) & upto & ~1u );
 					}
 				}
 				else
@@ -726,7 +731,9 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 #pragma unroll
 				for( int u = 0; u < U; u++ )
 				{
-					e[u] = seg_beg[kk[u]] + ( ff[u] - seg_pre[kk[u]] );
+					seg_rec const r = seg[kk[u]];
+					e[u] = r.off + ff[u];
+					sv[u] = r.src;
 					j[u] = ok[u] ? lds<( SPICE_NT & 4 ) != 0>( v.col + e[u] ) : 0;
 				}
 				if constexpr( eager )
@@ -744,7 +751,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 #pragma unroll
 					for( int u = 0; u < U; u++ )
 					{
-						bool const pl = ok[u] && catch_up && S::plastic( seg_src[kk[u]], j[u], info );
+						bool const pl = ok[u] && catch_up && S::plastic( sv[u], j[u], info );
 						f8[u] = pl ? std::uint8_t( v.fl8[prev + j[u]] | ( seg_pend[kk[u]] ? 0x80 : 0 ) ) : std::uint8_t( 0 );
 						yv[u] = pl && seg_pend[kk[u]] ? v.ytr[prev + j[u]] : 0.0f;
 					}
@@ -752,7 +759,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					for( int u = 0; u < U; u++ )
 					{
 						if( !ok[u] ) continue;
-						std::int32_t const s = seg_src[kk[u]];
+						std::int32_t const s = sv[u];
 						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
 						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc, stage };
 						reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
@@ -775,7 +782,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 				for( int u = 0; u < U; u++ )
 				{
 					if( !ok[u] ) continue;
-					std::int32_t const s = seg_src[kk[u]];
+					std::int32_t const s = sv[u];
 					tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
 					tile_iter<N, FIXED> dst{ &v, j[u], t0, acc, stage };
 					if constexpr( plastic )
