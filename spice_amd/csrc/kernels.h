@@ -604,6 +604,8 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	__shared__ std::int32_t seg_synced[plastic && !eager ? BS : 1];
 	__shared__ std::int32_t seg_pend[plastic ? BS : 1]; // eager: the row's spike also arrived at T - 1
 	__shared__ float seg_xf[eager ? BS : 1];            // eager: the row's pre trace after T - 1
+	__shared__ std::uint8_t tile_f8[eager ? TILE_MAX : 1]; // eager: the tile's step T - 1 flags
+	__shared__ float tile_y[eager ? TILE_MAX : 1];         // eager: the tile's post traces after T - 1
 	__shared__ std::uint64_t wsum[BS / 64];
 
 	std::int64_t const T = v.ctl->step;
@@ -632,6 +634,13 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
 		for( int x = tid; x < nacc * TILE_MAX; x += BS ) acc[x] = word( 0 );
+		if constexpr( eager )
+			if( catch_up ) // what step T - 1's deferred events read of their targets
+				for( int x = tid; x < tw; x += BS )
+				{
+					tile_f8[x] = v.fl8[prev + t0 + x];
+					tile_y[x] = v.ytr[prev + t0 + x];
+				}
 		// the tile's values of the attributes receive() reads (they do not change during
 		// delivery: the neuron update runs after it)
 		util::for_n<N::size>( [&]( auto I ) {
@@ -752,8 +761,8 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					for( int u = 0; u < U; u++ )
 					{
 						bool const pl = ok[u] && catch_up && S::plastic( sv[u], j[u], info );
-						f8[u] = pl ? std::uint8_t( v.fl8[prev + j[u]] | ( seg_pend[kk[u]] ? 0x80 : 0 ) ) : std::uint8_t( 0 );
-						yv[u] = pl && seg_pend[kk[u]] ? v.ytr[prev + j[u]] : 0.0f;
+						f8[u] = pl ? std::uint8_t( tile_f8[j[u] - t0] | ( seg_pend[kk[u]] ? 0x80 : 0 ) ) : std::uint8_t( 0 );
+						yv[u] = pl && seg_pend[kk[u]] ? tile_y[j[u] - t0] : 0.0f;
 					}
 #pragma unroll
 					for( int u = 0; u < U; u++ )
