@@ -6,6 +6,7 @@
 #include <spice/util/layout.h>
 #include <spice_hip.h>
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <limits>
@@ -648,10 +649,12 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			};
 			launch_cfg cfg = c->get_launch();
 			double best_t = time_of( cfg );
+			std::vector<std::pair<double, launch_cfg>> timed{ { best_t, cfg } };
 			std::int64_t const N = n->num_neurons();
 			launch_cfg const start = cfg;
 			auto trial = [&]( launch_cfg const & x ) {
 				double const t = time_of( x );
+				timed.push_back( { t, x } );
 				if( t < best_t )
 				{
 					best_t = t;
@@ -699,6 +702,30 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 						if( x.flush_slots == base.flush_slots && x.flush_grid == base.flush_grid ) continue;
 						trial( x );
 					}
+			}
+			// the copy keeps stepping, so each shape was timed on other activity (a bursting
+			// network's step time moves by 2x and more): the three fastest shapes are timed
+			// again, in turn, three times each, and the fastest in total is kept
+			std::sort( timed.begin(), timed.end(), []( auto const & a, auto const & b ) { return a.first < b.first; } );
+			std::vector<launch_cfg> top;
+			for( auto const & tx : timed )
+			{
+				launch_cfg const & x = tx.second;
+				bool seen = false;
+				for( launch_cfg const & y : top )
+					seen = seen || ( x.tile_w == y.tile_w && x.flush_slots == y.flush_slots && x.flush_grid == y.flush_grid &&
+					                 x.recv_block == y.recv_block && x.recv_parts == y.recv_parts );
+				if( !seen ) top.push_back( x );
+				if( top.size() == 3 ) break;
+			}
+			if( top.size() > 1 )
+			{
+				std::vector<double> sum( top.size(), 0.0 );
+				for( int r = 0; r < 3; r++ )
+					for( std::size_t i = 0; i < top.size(); i++ ) sum[i] += time_of( top[i] );
+				std::size_t const k = std::size_t( std::min_element( sum.begin(), sum.end() ) - sum.begin() );
+				cfg = top[k];
+				best_t = sum[k] / 3.0;
 			}
 			best = { cfg, best_t * 1e3 };
 			std::lock_guard<std::mutex> g( g_tune_mutex );
