@@ -737,13 +737,38 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 				}
 				else
 					segments_of<U>( seg_pre, nseg, ff, kk );
-#pragma unroll
-				for( int u = 0; u < U; u++ )
+				if constexpr( !plastic )
 				{
-					seg_rec const r = seg[kk[u]];
-					e[u] = r.off + ff[u];
-					sv[u] = r.src;
-					j[u] = ok[u] ? lds<( SPICE_NT & 4 ) != 0>( v.col + e[u] ) : 0;
+					// all U records, then all U target ids, without branches: an entry past the
+					// chunk's end reads the chunk's last entry (a valid address) and is ignored
+					// (plastic models keep the branches: the extra live registers cost them a wave
+					// per SIMD)
+					seg_rec r[U];
+#pragma unroll
+					for( int u = 0; u < U; u++ ) r[u] = seg[kk[u]];
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						e[u] = r[u].off + ( ok[u] ? ff[u] : total - 1 );
+						sv[u] = r[u].src;
+					}
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						std::int32_t const jj = lds<( SPICE_NT & 4 ) != 0>( v.col + e[u] );
+						j[u] = ok[u] ? jj : 0;
+					}
+				}
+				else
+				{
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						seg_rec const r = seg[kk[u]];
+						e[u] = r.off + ff[u];
+						sv[u] = r.src;
+						j[u] = ok[u] ? lds<( SPICE_NT & 4 ) != 0>( v.col + e[u] ) : 0;
+					}
 				}
 				if constexpr( eager )
 				{
