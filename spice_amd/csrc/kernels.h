@@ -449,13 +449,13 @@ struct tile_proxy
 	std::int32_t j;
 	A * lds; // accumulator of attribute I for this tile, indexed by j - tile start
 	std::int32_t t0;
-	std::uint32_t const * stage; // Staged: attribute I's values of the tile, by j - tile start
+	std::uint32_t sval; // Staged: attribute I's value (bits) from the tile's LDS copy
 	__device__ operator T() const
 	{
 		if constexpr( Staged )
 		{
 			using B = std::conditional_t<sizeof( T ) == 1, std::uint8_t, std::conditional_t<sizeof( T ) == 2, std::uint16_t, std::uint32_t>>;
-			return __builtin_bit_cast( T, B( stage[j - t0] ) );
+			return __builtin_bit_cast( T, B( sval ) );
 		}
 		else
 			return ld<T>( v->na[I], j );
@@ -493,8 +493,37 @@ struct tile_iter
 		constexpr int a = __builtin_popcount( mask & ( ( 1u << I ) - 1u ) );
 		constexpr int b = __builtin_popcount( smask & ( ( 1u << I ) - 1u ) );
 		using A = acc_type<T, FIXED>;
-		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, ( ( smask >> I ) & 1u ) != 0>{
-		    v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0, stage + b * TILE_MAX };
+		constexpr bool staged = ( ( smask >> I ) & 1u ) != 0;
+		std::uint32_t sv = 0;
+		if constexpr( staged ) sv = stage[b * TILE_MAX + ( j - t0 )];
+		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, staged>{ v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0, sv };
+	}
+	__device__ uint_ id() const { return uint_( j ); }
+};
+
+// The same target, with the staged attribute values already read (sval[k]: the k-th staged
+// attribute): the delivery of static models reads them for all entries of a pass at once.
+template <typename Neuron, bool FIXED>
+struct tile_iter_vals
+{
+	static constexpr int NS = __builtin_popcount( staged_mask<Neuron>() ) > 0 ? __builtin_popcount( staged_mask<Neuron>() ) : 1;
+	dview const * v;
+	std::int32_t j;
+	std::int32_t t0;
+	acc_word<Neuron, FIXED> * lds;
+	std::uint32_t sval[NS];
+	template <int I>
+	__device__ auto get() const
+	{
+		using T = typename Neuron::template type<I>;
+		constexpr std::uint32_t mask = receive_mask_v<Neuron>;
+		constexpr std::uint32_t smask = staged_mask<Neuron>();
+		constexpr int a = __builtin_popcount( mask & ( ( 1u << I ) - 1u ) );
+		constexpr int b = __builtin_popcount( smask & ( ( 1u << I ) - 1u ) );
+		using A = acc_type<T, FIXED>;
+		constexpr bool staged = ( ( smask >> I ) & 1u ) != 0;
+		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, staged>{ v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0,
+		                                                                  staged ? sval[b] : 0u };
 	}
 	__device__ uint_ id() const { return uint_( j ); }
 };
@@ -808,6 +837,29 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 							pev++;
 						}
 						N::receive( s, dst, itr, info, bak );
+					}
+				}
+				else if constexpr( !plastic )
+				{
+					// the staged attribute values of all U targets first (one LDS round), then
+					// receive() for each entry
+					using iter_v = tile_iter_vals<N, FIXED>;
+					std::uint32_t svals[U][iter_v::NS];
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+#pragma unroll
+						for( int b = 0; b < iter_v::NS; b++ )
+							svals[u][b] = nstaged > 0 ? stage[b * TILE_MAX + ( ok[u] ? j[u] - t0 : 0 )] : 0u;
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						if( !ok[u] ) continue;
+						std::int32_t const s = sv[u];
+						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
+						iter_v dst{ &v, j[u], t0, acc, {} };
+#pragma unroll
+						for( int b = 0; b < iter_v::NS; b++ ) dst.sval[b] = svals[u][b];
+						N::receive( s, dst, no_syn_iter{}, info, bak );
 					}
 				}
 				else
