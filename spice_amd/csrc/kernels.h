@@ -1020,6 +1020,35 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 	{
 		typename N::tuple_t s;
 		load_tuple<N>( s, v.na, i );
+		// the plastic bookkeeping this neuron's step reads does not depend on the update: load
+		// it now, with the state (after the state's store the loads would wait for the update)
+		std::int64_t const N64 = v.N;
+		std::int64_t const hcur = ( ( T & ( HISTORY - 1 ) ) >> 6 ) * N64 + i; // step T's history word
+		unsigned long long h_arr = 0, h_next = 0, h_now = 0;
+		float x_prev = 0.0f, y_prev = 0.0f;
+		std::uint8_t has_plastic = 0;
+		std::int64_t in_lo = 0, in_hi = 0;
+		std::int32_t synced0 = 0, pend0 = 0;
+		if constexpr( plastic )
+		{
+			if( T >= v.D ) h_arr = v.hring[( ( ( T - v.D ) & ( HISTORY - 1 ) ) >> 6 ) * N64 + i];
+			if( T + 1 >= v.D && v.D != 1 ) h_next = v.hring[( ( ( T + 1 - v.D ) & ( HISTORY - 1 ) ) >> 6 ) * N64 + i];
+			h_now = v.hring[hcur];
+			has_plastic = v.hasp[i];
+			if constexpr( eager )
+			{
+				std::int64_t const prv = N64 - ( T & 1 ) * N64;
+				x_prev = v.xtr[prv + i];
+				y_prev = v.ytr[prv + i];
+				in_lo = v.tp_ptr[i];
+				in_hi = v.tp_ptr[i + 1];
+			}
+			else
+			{
+				synced0 = v.synced[i];
+				pend0 = v.pend[i];
+			}
+		}
 		util::for_n<N::size>( [&]( auto I ) {
 			if constexpr( ( ( rmask >> I ) & 1u ) != 0 )
 			{
@@ -1046,36 +1075,33 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 
 		if constexpr( plastic )
 		{
-			std::int64_t const N64 = v.N;
 			// this neuron's spike of step T - D reaches its synapses now
-			bool const arrived = T >= v.D && spiked_at( v, std::int32_t( i ), std::int32_t( T - v.D ) );
+			bool const arrived = T >= v.D && ( ( h_arr >> ( ( T - v.D ) & 63 ) ) & 1ull );
 			// and its spike of step T + 1 - D at the next step (recorded already unless D = 1)
-			bool const arrives_next = T + 1 >= v.D && ( v.D == 1 ? spiked : spiked_at( v, std::int32_t( i ), std::int32_t( T + 1 - v.D ) ) );
+			bool const arrives_next = T + 1 >= v.D && ( v.D == 1 ? spiked : ( ( h_next >> ( ( T + 1 - v.D ) & 63 ) ) & 1ull ) != 0 );
 			{ // record this step in the history ring (overwriting step T - HISTORY)
-				std::int64_t const w = ( T & ( HISTORY - 1 ) ) >> 6;
 				unsigned long long const bit = 1ull << ( T & 63 );
-				unsigned long long const word = v.hring[w * N64 + i];
-				unsigned long long const nw = spiked ? ( word | bit ) : ( word & ~bit );
-				if( nw != word ) v.hring[w * N64 + i] = nw;
+				unsigned long long const nw = spiked ? ( h_now | bit ) : ( h_now & ~bit );
+				if( nw != h_now ) v.hring[hcur] = nw;
 			}
 			if constexpr( eager )
 			{
 				// traces after step T (the dense checker's per-step recurrence) and the step's
 				// flags, into the arrays of step parity T & 1
-				std::int64_t const cur = ( T & 1 ) * N64, prv = N64 - cur;
+				std::int64_t const cur = ( T & 1 ) * N64;
 				float const d = S::trace_decay( dt );
-				v.xtr[cur + i] = fmaf( v.xtr[prv + i], d, arrived ? 1.0f : 0.0f );
-				v.ytr[cur + i] = fmaf( v.ytr[prv + i], d, spiked ? 1.0f : 0.0f );
+				v.xtr[cur + i] = fmaf( x_prev, d, arrived ? 1.0f : 0.0f );
+				v.ytr[cur + i] = fmaf( y_prev, d, spiked ? 1.0f : 0.0f );
 				v.fl8[cur + i] = std::uint8_t( ( spiked ? F8_SPIKED : 0 ) | ( arrived ? F8_ARRIVED : 0 ) | ( arrives_next ? F8_ARRIVES_NEXT : 0 ) );
 				// pre pass: a row whose spike arrives now, unless it arrives again at T + 1 (then
 				// that delivery applies this step's events of the row); post pass: a spiking
 				// neuron with incoming plastic synapses
-				flush = arrived && !arrives_next && v.hasp[i];
-				post = spiked && v.tp_ptr[i + 1] > v.tp_ptr[i];
+				flush = arrived && !arrives_next && has_plastic;
+				post = spiked && in_hi > in_lo;
 			}
 			else
 			{
-				std::int32_t synced = v.synced[i], pend = v.pend[i];
+				std::int32_t synced = synced0, pend = pend0;
 				bool dirty = false;
 				if( arrived )
 				{
@@ -1093,7 +1119,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 				//  * their history window is full.
 				// A row whose spike arrives at T + 1 is left to that delivery instead: the flush
 				// of step T may run while step T + 1 delivers, so their rows must differ.
-				if( v.hasp[i] && !arrives_next && ( pend || T - synced >= age || ( synced < 0 && T >= i % age ) ) )
+				if( has_plastic && !arrives_next && ( pend || T - synced >= age || ( synced < 0 && T >= i % age ) ) )
 				{
 					flush = true;
 					fl_synced = synced;
