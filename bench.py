@@ -359,7 +359,8 @@ def cpu_baseline(model, lay, delay, seed, net, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10_000, help="timed steps (10,000 = 1 s bio)")
+    ap.add_argument("--steps", type=int, default=100_000,
+                    help="timed steps (100,000 = 10 s bio: the configuration's and the reference's benchmark length)")
     ap.add_argument("--warmup", type=int, default=2_000)
     ap.add_argument("--workload", default="brunel_stdp_1e9", choices=sorted(WORKLOADS))
     ap.add_argument("--profile-steps", type=int, default=1_000)
