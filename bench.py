@@ -408,6 +408,14 @@ def main():
 
     net.run(args.warmup)
     net.synchronize()
+
+    # Roofline: per-kernel mean launch time from HIP events on the net's stream, over the
+    # profile steps right after the warmup (so that a profiler run of the same command with
+    # fewer timed steps measures the same steps of the network)
+    net.reset_stats()
+    ms = net.run_profiled(args.profile_steps)
+    pst = net.stats()
+    net.synchronize()
     net.reset_stats()
 
     t0 = time.perf_counter()
@@ -417,11 +425,6 @@ def main():
     st = net.stats()
     bio = args.steps * dt
     print(f"[bench] timed {args.steps} steps in {wall:.3f}s, stats {st}", file=sys.stderr, flush=True)
-
-    # Roofline: per-kernel mean launch time from HIP events on the net's stream.
-    net.reset_stats()
-    ms = net.run_profiled(args.profile_steps)
-    pst = net.stats()
     bytes_per = algorithmic_bytes(model, n, pst, args.profile_steps)
     kern = max(ms, key=lambda k: ms[k])
     avg_ms = ms[kern] / args.profile_steps
