@@ -12,8 +12,8 @@ plasticity, neuron update, plasticity flush). The timed region replays captured 
 of steps with all state resident in HBM; value = wall seconds / simulated (bio) seconds.
 
 Printed: one JSON line with the metric, synapse events/s, a roofline object for the
-dominant kernel (algorithmic bytes per launch / mean launch time from HIP events on the
-net's stream) and a cpu_baseline object (the single-threaded CPU checker, a restatement of
+dominant kernel (algorithmic bytes per launch / mean kernel duration from HIP events attached
+to each dispatch on the net's stream) and a cpu_baseline object (the single-threaded CPU checker, a restatement of
 the reference's cpu::snn, timed on this host on a bounded number of steps of the same net).
 """
 import argparse
@@ -409,9 +409,9 @@ def main():
     net.run(args.warmup)
     net.synchronize()
 
-    # Roofline: per-kernel mean launch time from HIP events on the net's stream, over the
-    # profile steps right after the warmup (so that a profiler run of the same command with
-    # fewer timed steps measures the same steps of the network)
+    # Roofline: per-kernel mean duration from HIP events attached to each dispatch on the net's
+    # stream, over the profile steps right after the warmup (so that a profiler run of the
+    # same command with fewer timed steps measures the same steps of the network)
     net.reset_stats()
     ms = net.run_profiled(args.profile_steps)
     pst = net.stats()

@@ -217,8 +217,9 @@ int spice_hip_copy_spikes( spice_net * net, int32_t * out, uint64_t cap, uint64_
  * step sequence. Equivalent to nsteps calls of spice_hip_step. Asynchronous. */
 int spice_hip_run( spice_net * net, int64_t nsteps );
 
-/* Enqueues nsteps steps without graph capture, with HIP events recorded around every kernel
- * on the net's stream, waits, and adds the summed per-kernel device times in milliseconds to
+/* Enqueues nsteps steps without graph capture, each kernel launched with start and stop HIP
+ * events attached to its dispatch (hipExtLaunchKernel: the kernel's own duration, without the
+ * launch gap) on the net's stream, waits, and adds the summed per-kernel device times in milliseconds to
  * kernel_ms[0] (spike delivery; for plastic models including the previous step's plasticity
  * work, which runs in the same kernel), kernel_ms[1] (neuron update) and kernel_ms[2]
  * (always 0: no separate plasticity kernel). Same results as spice_hip_run. Used for roofline

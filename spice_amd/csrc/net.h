@@ -168,9 +168,9 @@ public:
 		while( done < n ) eager();
 	}
 
-	// Runs n steps eagerly with HIP events around every kernel on the net's stream and adds
-	// the summed kernel times (ms) to kernel_ms[0..2] = receive, update, flush (lazy
-	// plasticity; eager plasticity runs inside the receive kernel).
+	// Runs n steps eagerly, each kernel with HIP events attached to its dispatch on the net's
+	// stream, and adds the summed kernel durations (ms) to kernel_ms[0..2] = receive, update,
+	// plasticity (0: plasticity runs inside the receive kernel).
 	void run_profiled( std::int64_t n, double * kernel_ms ) override
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
@@ -197,11 +197,10 @@ public:
 				for( int i = 0; i < k; i++ )
 				{
 					hipEvent_t * e = &ev[i * ev_per_step];
-					for( int q = 0; q < 3; q++ )
+					for( int q = 0; q < 2; q++ ) // kernel_ms[2] stays 0: plasticity runs inside the delivery kernel
 					{
-						if( q == 2 ) continue; // plasticity runs inside the delivery kernel
 						float ms = 0;
-						SPICE_HIP( hipEventElapsedTime( &ms, e[q], e[q + 1] ) );
+						SPICE_HIP( hipEventElapsedTime( &ms, e[2 * q], e[2 * q + 1] ) );
 						kernel_ms[q] += ms;
 					}
 				}
@@ -874,15 +873,29 @@ private:
 			SPICE_LAUNCH( k_receive_tiled<Model, F, 1024><<<grid, 1024, 0, _s>>>( v, _grid_tiles ) );
 	}
 
+	// Launches `fn` on the net's stream with `start`/`stop` attached to the dispatch itself, so
+	// that their elapsed time is the kernel's own duration (no launch gap, unlike events
+	// recorded between launches). args: the kernel's exact parameter types.
+	template <typename... A>
+	void launch_timed( void const * fn, unsigned grid, unsigned block, hipEvent_t start, hipEvent_t stop, A... a )
+	{
+		void * args[] = { static_cast<void *>( &a )... };
+		SPICE_HIP( hipExtLaunchKernel( fn, dim3( grid ), dim3( block ), args, 0, _s, start, stop, 0 ) );
+	}
+
+	// e[0], e[1] time the delivery kernel, e[2], e[3] the neuron update
 	template <bool F>
 	void launch_step_events( hipEvent_t * e, dview const & v )
 	{
-		SPICE_HIP( hipEventRecord( e[0], _s ) );
-		launch_receive<F>( v );
-		SPICE_HIP( hipEventRecord( e[1], _s ) );
-		SPICE_LAUNCH( k_update<Model, F><<<_grid_upd, SPICE_UPDATE_BLOCK, 0, _s>>>( v ) );
-		SPICE_HIP( hipEventRecord( e[2], _s ) );
-		SPICE_HIP( hipEventRecord( e[3], _s ) );
+		unsigned const grid = unsigned( _grid_tiles + _grid_plast );
+		int const tiles = _grid_tiles;
+		if( _lc.recv_block == 256 )
+			launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, 256> ), grid, 256, e[0], e[1], v, tiles );
+		else if( _lc.recv_block == 512 )
+			launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, 512> ), grid, 512, e[0], e[1], v, tiles );
+		else
+			launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, 1024> ), grid, 1024, e[0], e[1], v, tiles );
+		launch_timed( reinterpret_cast<void const *>( &k_update<Model, F> ), unsigned( _grid_upd ), SPICE_UPDATE_BLOCK, e[2], e[3], v );
 	}
 
 	void capture()

@@ -193,8 +193,8 @@ class Snn:
         _lib.check(_lib.lib().spice_hip_run(self._h, int(nsteps)))
 
     def run_profiled(self, nsteps):
-        """Runs nsteps eagerly with HIP events around each kernel; returns summed device ms
-        per kernel: {"receive": ..., "update": ..., "flush": ...} ("receive" includes the
+        """Runs nsteps eagerly, each kernel launched with HIP events attached to its dispatch
+        (its own duration, without the launch gap); returns summed device ms per kernel: {"receive": ..., "update": ..., "flush": ...} ("receive" includes the
         previous step's plasticity work of plastic models, which runs in the same kernel;
         "flush" is 0)."""
         ms = (_c.c_double * 3)(0.0, 0.0, 0.0)

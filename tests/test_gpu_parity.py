@@ -192,6 +192,27 @@ def test_run_graph_equals_step(gpu):
     assert a.stats() == b.stats()
 
 
+@pytest.mark.parametrize("launch", [(256, 128, 1024, 256, 1), (256, 128, 1024, 512, 2), (256, 128, 1024, 1024, 1)],
+                         ids=["block256", "block512", "block1024"])
+def test_run_profiled_equals_run(gpu, launch):
+    """spice_hip_run_profiled (each kernel launched with events on its dispatch) steps the net
+    exactly as spice_hip_run, at each delivery workgroup size, and reports positive kernel
+    times (plastic model: plasticity inside the delivery kernel, so "flush" is 0)."""
+    lay = three_pop_layout(2000)
+    a = Snn("brunel_3pop_stdp", Layout(*lay), 1e-4, 15, seed=9, fixed_point=True)
+    b = Snn("brunel_3pop_stdp", Layout(*lay), 1e-4, 15, seed=9, fixed_point=True)
+    a.set_launch(*launch)
+    b.set_launch(*launch)
+    a.run(40)
+    ms = a.run_profiled(150)
+    a.run(10)
+    b.run(200)
+    assert a.clock() == b.clock()
+    assert_state_equal(a.neurons(), b.neurons(), "profiled vs run")
+    assert_state_equal(a.synapses(ell=False), b.synapses(ell=False), "weights")
+    assert ms["receive"] > 0 and ms["update"] > 0 and ms["flush"] == 0
+
+
 def test_cpp_drop_in_api(gpu):
     """The public C++ API (spice::hip::snn<Model>, include/spice/hip/snn.h) driven by a C++
     program that mirrors the reference's dSNN tests (spice_test/cuda/snn.d.cpp:72-157)."""
