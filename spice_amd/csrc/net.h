@@ -176,10 +176,10 @@ public:
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
 		constexpr int batch = 128, ev_per_step = 4;
 		synchronize(); // profiled steps run serially on the main stream
-		std::vector<hipEvent_t> ev( batch * ev_per_step );
-		for( auto & e : ev ) SPICE_HIP( hipEventCreate( &e ) );
+		std::vector<hipEvent_t> ev( batch * ev_per_step, nullptr );
 		try
 		{
+			for( auto & e : ev ) SPICE_HIP( hipEventCreate( &e ) );
 			for( std::int64_t done = 0; done < n; )
 			{
 				int const k = int( std::min<std::int64_t>( batch, n - done ) );
@@ -209,7 +209,8 @@ public:
 		}
 		catch( ... )
 		{
-			for( auto & e : ev ) (void)hipEventDestroy( e );
+			for( auto & e : ev )
+				if( e ) (void)hipEventDestroy( e );
 			throw;
 		}
 		for( auto & e : ev ) (void)hipEventDestroy( e );
