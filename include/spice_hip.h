@@ -266,7 +266,7 @@ int spice_hip_get_launch( spice_net * net, spice_launch * out );
 int spice_hip_set_launch( spice_net * net, const spice_launch * in );
 /* Times candidate launch shapes on a copy of the network (warmup steps first, then `steps`
  * graph-replayed steps per candidate, wall time per step: plasticity overlaps delivery, so
- * shapes are judged together; tile width first, then delivery parts, then the plasticity
+ * shapes are evaluated together; tile width first, then delivery parts, then the plasticity
  * shape; the three fastest are then timed again in turn, three times each, since the copy's
  * activity drifts between candidates),
  * applies the fastest to the network and returns it with its step time (us). The network

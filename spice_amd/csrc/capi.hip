@@ -637,7 +637,7 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			c->run( warmup );
 			c->synchronize();
 			// wall time per step (ms) of `steps` graph-replayed steps under cfg: a step's
-			// plasticity work runs beside the next step's delivery, so shapes are judged together
+			// plasticity work runs beside the next step's delivery, so shapes are evaluated together
 			auto time_of = [&]( launch_cfg cfg ) {
 				c->set_launch( cfg );
 				c->run( 32 ); // capture + warm the graph

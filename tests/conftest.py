@@ -30,5 +30,5 @@ def gpu():
     if not _gpu_present():
         pytest.skip("no GPU")
     lib = os.path.join(ROOT, "spice_amd", "libspice_hip.so")
-    assert os.path.exists(lib), "libspice_hip.so missing on a GPU box: build it first"
+    assert os.path.exists(lib), "libspice_hip.so missing on a machine with a GPU: build it first"
     return True
