@@ -9,7 +9,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _ROOT = os.path.dirname(_HERE)
-ORACLE_LIB = os.path.join(_HERE, "liboracle.so")
+# SPICE_ORACLE_LIB: another build of the checker (the mutation tests load deliberately broken
+# builds made from edited copies of the model headers)
+ORACLE_LIB = os.environ.get("SPICE_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 REF_LIB = os.path.join(_HERE, "_ref", "libspice_ref.so")
 REF_IEEE_LIB = os.path.join(_HERE, "_ref", "libspice_ref_ieee.so")
 

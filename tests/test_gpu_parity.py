@@ -300,3 +300,18 @@ def test_trace_stdp_graph_replay_bit_exact(gpu, delay):
         assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), f"weights after {500 * (k + 1)} steps")
     st = d.stats()
     assert st["plastic_events"] > 0 and st["flush_rows"] > 0 and st["flush_synapses"] > 0
+
+
+@pytest.mark.parametrize("model", ["brunel_3pop", "brunel_3pop_stdp"])
+@pytest.mark.parametrize("fixed", [False, True])
+def test_lif_closed_form_on_gpu(gpu, model, fixed):
+    """The headline model's neuron on the GPU meets the closed forms of
+    tests/test_lif_closed_form.py (threshold, exact reset, refractory period, tau, 5/N and
+    -25/N weights, delivery delay, the first STDP pre event)."""
+    from tests.test_lif_closed_form import check_lif_closed_form
+
+    def make(m, layout, dt, delay):
+        return Snn(m, Layout(*layout), dt, delay, seed=3, fixed_point=fixed)
+
+    for delay in (1, 15):
+        check_lif_closed_form(make, model, delay).close()
