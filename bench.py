@@ -52,10 +52,18 @@ WORKLOADS = {
     "ref_brunel_1e9": ("brunel", 141_421, 0.1, 15),
     "ref_vogels_1e9": ("vogels_abbott", 223_606, 0.02, 8),
     "ref_brunel_plus_6e8": ("brunel_with_plasticity", 109_544, 0.1, 15),
+    # synth (samples/main.cpp:1285-1291: one population, N = sqrt(S / p)): the published
+    # simulation row (p = 0.00156, delay 1) and the setup-time row (p = 0.05, delay 1)
+    "ref_synth_1e9": ("synth", 800_640, 0.00156, 1),
+    "ref_synth_setup_1e9": ("synth", 141_421, 0.05, 1),
 }
 # wall/bio the reference publishes for those networks on one GPU (results/spice.json:218-225,
-# 122-129, 378-385; GPU model not stated)
-PUBLISHED = {"ref_brunel_1e9": 0.98772, "ref_vogels_1e9": 1.08669, "ref_brunel_plus_6e8": 21.8024}
+# 122-129, 378-385, 26-33; GPU model not stated)
+PUBLISHED = {"ref_brunel_1e9": 0.98772, "ref_vogels_1e9": 1.08669, "ref_brunel_plus_6e8": 21.8024,
+             "ref_synth_1e9": 1.39947}
+# setup time (graph generation + initial state, device synchronised) the reference publishes
+# (results/spice.json:1274-1281, method samples/main.cpp:1337-1348)
+PUBLISHED_SETUP = {"ref_synth_setup_1e9": 0.253356}
 
 
 def workload_layout(name):
@@ -135,13 +143,73 @@ def sweep_nets(count, seed):
     return nets
 
 
+def stat_pops(model, pops):
+    """Populations for rate / ISI CV statistics: the layout's, except Vogels-Abbott's single
+    layout population, which the model splits into excitatory (ids below 0.8 N) and inhibitory."""
+    if model == "vogels_abbott" and len(pops) == 1:
+        nexc = int(np.float32(0.8) * np.float32(pops[0]))
+        return [nexc, pops[0] - nexc]
+    return list(pops)
+
+
+def _served_rasters(jobs, fixed, together):
+    """Rasters (and job records) of `jobs` = [(model, pops, conns, delay, seed, params, steps)]
+    streamed by the library's server: all jobs in one server (stepped together in batched
+    chunks) or each job in a server of its own."""
+    from spice_amd import Layout, Server
+    from spice_amd.raster import Raster
+    rasters, recs = {}, []
+    groups = [list(range(len(jobs)))] if together else [[k] for k in range(len(jobs))]
+    for g in groups:
+        ids = {}
+
+        def sink(job, first, off, ids_, _ids=ids):
+            rasters[_ids[job]].add(first, off, ids_)
+
+        with Server(budget_bytes=250e9, chunk_steps=1000, on_spikes=sink) as sv:
+            for k in g:
+                m, pops, conns, d, seed, par, steps = jobs[k]
+                rasters[k] = Raster(sum(pops), stat_pops(m, pops))
+                ids[sv.submit(m, Layout(pops, conns), 1e-4, steps, delay=d, seed=seed, params=par,
+                              fixed_point=fixed)] = k
+            for i in list(ids):
+                recs.append((ids[i], sv.wait(i)))
+    return rasters, dict(recs)
+
+
+def compare_rasters(a, b, dt, min_spikes):
+    """Per-net, per-population rate and mean ISI CV of rasters b against a (relative
+    differences): the worst net overall and among nets with >= min_spikes spikes, and the
+    share of compared nets whose every population is within 2 %."""
+    rows = []
+    for k in a:
+        ra, rb = np.array(a[k].rates(dt)), np.array(b[k].rates(dt))
+        ca, cb = np.array(a[k].isi_cv()), np.array(b[k].isi_cv())
+        dr = np.abs(rb - ra) / np.maximum(ra, 1e-9)
+        ok = np.isfinite(ca) & np.isfinite(cb)
+        dc = np.abs(cb[ok] - ca[ok]) / np.maximum(ca[ok], 1e-9)
+        rows.append((k, int(a[k].count.sum()), float(dr.max(initial=0.0)), float(dc.max(initial=0.0))))
+    big = [r for r in rows if r[1] >= min_spikes]
+    worst = max(rows, key=lambda r: max(r[2], r[3]), default=(None, 0, 0.0, 0.0))
+    worst_big = max(big, key=lambda r: max(r[2], r[3]), default=(None, 0, 0.0, 0.0))
+    return {"nets": len(rows), "max_rate_rel_diff": round(worst[2], 5), "max_isi_cv_rel_diff": round(worst[3], 5),
+            "worst_net": worst[0], "worst_net_spikes": worst[1],
+            f"nets_with_{min_spikes}_spikes": len(big),
+            f"max_rate_rel_diff_{min_spikes}": round(worst_big[2], 5),
+            f"max_isi_cv_rel_diff_{min_spikes}": round(worst_big[3], 5),
+            "within_2pct_share": round(sum(1 for r in rows if r[2] <= 0.02 and r[3] <= 0.02) / max(1, len(rows)), 4),
+            f"within_2pct_share_{min_spikes}": round(sum(1 for r in big if r[2] <= 0.02 and r[3] <= 0.02)
+                                                     / max(1, len(big)), 4)}
+
+
 def run_batched(count, steps, seq_stride):
     """Aggregate synapse events/s of the sweep stepped as batches (one batch per model, the
     two batches on their own streams) vs. the same nets stepped one at a time on the same GPU
-    (every seq_stride-th net, rate = their events / their summed wall time); per-net spike
-    counts of the two runs are compared: fp32 accumulation order differs between launch shapes,
-    so trajectories diverge and only statistics agree -- summed over the sample (2 % tolerance),
-    per net for nets with >= 1e4 spikes, and as a z-score |a-b|/sqrt(a+b) per net."""
+    (every seq_stride-th net, rate = their events / their summed wall time). Statistics of that
+    sample: per net and population, firing rate and mean ISI CV with the nets stepped together
+    in batched chunks (spikes streamed by the server) against each net stepped alone -- in fp32,
+    where summation order differs and trajectories diverge, so only statistics agree, and in
+    fixed point, where the spike streams must be identical."""
     from spice_amd import Batch, Layout, Snn
     nets = sweep_nets(count, 7)
     dt = 1e-4
@@ -170,8 +238,9 @@ def run_batched(count, steps, seq_stride):
     for b in batches.values():
         b.close()
     # sequential plain runs of a sample
-    seq_events, seq_wall, rel = 0, 0.0, []
-    for k in range(0, count, seq_stride):
+    sample = list(range(0, count, seq_stride))
+    seq_events, seq_wall = 0, 0.0
+    for k in sample:
         m, pops, conns, d, seed, par = nets[k]
         net = Snn(m, Layout(pops, conns), dt, d, seed=seed, params=par)
         net.synchronize()
@@ -179,22 +248,32 @@ def run_batched(count, steps, seq_stride):
         net.run(steps)
         net.synchronize()
         seq_wall += time.perf_counter() - t1
-        st = net.stats()
-        seq_events += st["events"]
-        rel.append((st["spikes"], spikes[k]))
+        seq_events += net.stats()["events"]
         net.close()
     agg = events / wall
     seq = seq_events / seq_wall
     print(f"[bench] batched {count} nets ({syn:.3g} synapses) x {steps} steps: {wall:.3f}s, "
           f"{agg:.3g} ev/s; sequential sample {seq:.3g} ev/s", file=sys.stderr, flush=True)
+    jobs = [nets[k] + (steps,) for k in sample]
+    t1 = time.perf_counter()
+    together, _ = _served_rasters(jobs, False, True)
+    alone, _ = _served_rasters(jobs, False, False)
+    stats_fp32 = compare_rasters(alone, together, dt, 10_000)
+    _, fx_together = _served_rasters(jobs, True, True)
+    _, fx_alone = _served_rasters(jobs, True, False)
+    fx_same = all(fx_together[k]["digest"] == fx_alone[k]["digest"] and fx_together[k]["spikes"] == fx_alone[k]["spikes"]
+                  for k in range(len(jobs)))
+    print(f"[bench] sweep statistics ({len(jobs)} nets, {time.perf_counter() - t1:.1f}s): {stats_fp32}; "
+          f"fixed point identical: {fx_same}", file=sys.stderr, flush=True)
     return {"nets": count, "synapses": syn, "steps": steps, "bio_s": steps * dt, "setup_s": round(setup, 3),
             "wall_s": round(wall, 4), "syn_events_per_s": round(agg, 1),
             "sequential_syn_events_per_s": round(seq, 1), "speedup_vs_sequential": round(agg / seq, 3),
-            "sequential_sample": f"every {seq_stride}th net ({len(range(0, count, seq_stride))} nets), same steps",
-            "spike_count_rel_diff_total": round(abs(sum(a for a, _ in rel) - sum(b for _, b in rel))
-                                                / max(1, sum(a for a, _ in rel)), 5),
-            "max_spike_count_rel_diff_1e4": round(max([abs(a - b) / a for a, b in rel if a >= 10_000] or [0]), 5),
-            "max_spike_count_z": round(max([abs(a - b) / math.sqrt(a + b) for a, b in rel if a + b > 0] or [0]), 3)}
+            "sequential_sample": f"every {seq_stride}th net ({len(sample)} nets), same steps",
+            "spikes_total": int(sum(spikes.values())),
+            "statistics_sample": f"the same {len(sample)} nets, {steps} steps: stepped together in batched chunks "
+                                 f"(server) vs each alone; per net and population",
+            "statistics_fp32": stats_fp32,
+            "fixed_point_identical": fx_same}
 
 
 def serve_stream(count, rate, seed):
@@ -257,6 +336,7 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
     sv.close()
     failed = [r for r in recs if r["status"] != "done"]
     lat = np.array([r["t_done"] - r["t_submit"] for r in recs])
+    qdelay = np.array([r["t_admit"] - r["t_submit"] for r in recs])  # waiting for admission (budget)
     events = sum(r["events"] for r in recs)
     syn = sum(r["synapses"] for r in recs)
     print(f"[bench] served {count} jobs ({syn:.3g} synapses, {events:.3g} events) in {makespan:.1f}s: "
@@ -303,6 +383,10 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
         "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
         "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
         "latency_max_s": round(float(lat.max()), 4),
+        "admission_delay_p50_s": round(float(np.percentile(qdelay, 50)), 4),
+        "admission_delay_p99_s": round(float(np.percentile(qdelay, 99)), 4),
+        "admission_delay_max_s": round(float(qdelay.max()), 4),
+        "jobs_waiting_over_1s_for_admission": int(np.count_nonzero(qdelay > 1.0)),
         "syn_events_per_s": round(served, 1),
         "sequential_syn_events_per_s": round(seq, 1),
         "sequential_syn_events_per_s_incl_setup": round(seq_incl, 1),
@@ -368,7 +452,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--flags", type=int, default=0, help="spice_flag bits (1: int64 fixed-point accumulation; see include/spice_hip.h)")
     ap.add_argument("--batch-nets", type=int, default=512, help="nets of the batched sweep (0 = skip)")
-    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--batch-steps", type=int, default=10_000, help="steps of the sweep (10,000 = 1 s bio)")
     ap.add_argument("--batch-seq-stride", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--launch", default=None,
@@ -474,6 +558,8 @@ def main():
         "spikes_per_step": round(st["spikes"] / args.steps, 2),
         "events_per_step": round(st["events"] / args.steps, 1),
         "setup_s": round(setup_s, 4),
+        "setup_vs_published": (round(setup_s / PUBLISHED_SETUP[args.workload], 4)
+                               if args.workload in PUBLISHED_SETUP else None),
         "launch": net_launch,
         "autotune": tune,
         "roofline": roofline,

@@ -196,6 +196,26 @@ int spice_hip_create_from_csr(
     uint32_t flags,
     spice_net ** out_net );
 
+/* Replaces the reference's kernel-layer generate_adj_list (spice/cuda/algorithm.h:15-20,
+ * spice/cuda/algorithm.cu:397-443): generates the random graph of a layout (the generator of
+ * spice_hip_create, same seed -> same graph) into caller-owned DEVICE memory, as CSR:
+ * d_row_ptr receives num_neurons + 1 int64 offsets; d_col (NULL: offsets only) receives the
+ * nnz int32 targets if nnz <= col_cap (elements), else the call fails with
+ * SPICE_ERR_INVALID_ARGUMENT after setting *out_nnz (so a caller can size col and call again).
+ * *out_width: the layout's ELL width. The generator's scratch (connection table, scan partial
+ * sums) is allocated for the call. Synchronous; runs on the null stream. */
+int spice_hip_generate_graph(
+    const uint64_t * pops,
+    int32_t npops,
+    const spice_connection * conns,
+    int32_t nconns,
+    uint64_t seed,
+    int64_t * d_row_ptr,
+    int32_t * d_col,
+    uint64_t col_cap,
+    uint64_t * out_nnz,
+    uint64_t * out_width );
+
 /* Releases every device buffer of the net (the reference's dbuffer RAII,
  * spice/cuda/util/dbuffer.h:17-90). */
 int spice_hip_destroy( spice_net * net );
@@ -375,6 +395,15 @@ int spice_hip_export_adj( spice_net * net, int32_t * out, uint64_t cap );
 
 /* CSR adjacency: row_ptr (num_neurons + 1) and col (num_edges). */
 int spice_hip_export_csr( spice_net * net, int64_t * row_ptr, int32_t * col );
+
+/* Rows [row_first, row_last) of the CSR adjacency, for graphs too large to copy whole
+ * (1e10 synapses are 40 GB): row_ptr receives the row_last - row_first + 1 absolute offsets
+ * row_ptr[row_first] .. row_ptr[row_last]; col (NULL: offsets only) receives the
+ * row_ptr[row_last] - row_ptr[row_first] target ids of those rows, col_cap is its size in
+ * elements (SPICE_ERR_INVALID_ARGUMENT if too small). Synchronous. No counterpart in the
+ * reference, whose adj() copies the whole ELL array (spice/cuda/snn.d.cpp:308-312). */
+int spice_hip_export_csr_rows(
+    spice_net * net, uint64_t row_first, uint64_t row_last, int64_t * row_ptr, int32_t * col, uint64_t col_cap );
 
 /* neurons(): one attribute at a time, num_neurons elements (spice/snn.h:33,
  * spice/util/meta.h:212-233). `bytes` must equal num_neurons * attribute size. */

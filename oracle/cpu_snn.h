@@ -174,6 +174,7 @@ struct net_base
 	virtual bool export_traces( float * x, float * y ) const = 0;
 	virtual void set_params( float const * p ) = 0;
 	virtual void set_clock( std::int64_t i, float sum, float c ) = 0;
+	virtual void set_in_flight( std::vector<std::int64_t> const & counts, std::vector<int> const & ids ) = 0;
 };
 
 template <typename Model>
@@ -362,6 +363,30 @@ public:
 	{
 		_i = i;
 		_clock.set( sum, c );
+	}
+
+	// The spikes of the last counts.size() (<= delay) steps, oldest first: what the next
+	// steps deliver (and, for plastic models, their pre flags).
+	void set_in_flight( std::vector<std::int64_t> const & counts, std::vector<int> const & ids ) override
+	{
+		std::int64_t const k = std::int64_t( counts.size() );
+		if( k > _delay ) throw std::invalid_argument( "at most `delay` steps are in flight" );
+		for( int x : ids )
+			if( x < 0 || x >= _n ) throw std::invalid_argument( "spike id out of range" );
+		_counts = counts;
+		_ids = ids;
+		if constexpr( S::size > 0 )
+		{
+			std::size_t o = 0;
+			for( std::int64_t s = 0; s < k; s++ )
+			{
+				std::int64_t const step = _i - k + s;
+				auto & f = _flags[std::size_t( ( step % ( _delay + 1 ) + _delay + 1 ) % ( _delay + 1 ) )];
+				std::fill( f.begin(), f.end(), 0 );
+				for( std::int64_t q = 0; q < counts[s]; q++ ) f[ids[o + q]] = 1;
+				o += std::size_t( counts[s] );
+			}
+		}
 	}
 
 private:

@@ -42,55 +42,76 @@ struct csr
 	std::vector<std::int32_t> col;
 };
 
+// Length of row r: the sum over covering connections of the sampled degrees, each clipped to
+// what is left of the row width.
+inline std::int64_t row_length( std::vector<util::gen::conn> const & conns, std::int64_t r, std::int64_t width, std::uint64_t seed )
+{
+	std::int64_t total = 0;
+	for( std::uint32_t c = 0; c < conns.size(); c++ )
+	{
+		if( r < conns[c].src_first || r >= conns[c].src_last ) continue;
+		std::int64_t d = util::gen::sample_degree( seed, std::uint32_t( r ), c, conns[c] );
+		if( d > width - total ) d = width - total;
+		total += d;
+	}
+	return total;
+}
+
+// Writes the targets of row r to out[0 .. row_length): per covering connection, the sorted
+// targets of its exponential spacings.
+inline void row_targets( std::vector<util::gen::conn> const & conns, std::int64_t r, std::int64_t width, std::uint64_t seed, std::int32_t * out )
+{
+	std::int64_t off = 0, total = 0;
+	for( std::uint32_t c = 0; c < conns.size(); c++ )
+	{
+		auto const & cn = conns[c];
+		if( r < cn.src_first || r >= cn.src_last ) continue;
+		std::int64_t d = util::gen::sample_degree( seed, std::uint32_t( r ), c, cn );
+		if( d > width - total ) d = width - total;
+		total += d;
+		std::int32_t const range = cn.dst_last - cn.dst_first;
+		std::int32_t const R = range - std::int32_t( d );
+		// F_0 .. F_d (d + 1 draws), 4 per Philox block.
+		std::vector<std::uint32_t> f( ( d + 1 + 3 ) / 4 * 4 );
+		for( std::uint32_t b = 0; b * 4 < f.size(); b++ )
+			util::gen::exp_fix4( seed, std::uint32_t( r ), c, b, &f[b * 4] );
+		std::uint64_t ttotal = 0;
+		for( std::int64_t k = 0; k <= d; k++ ) ttotal += f[k];
+		std::uint64_t tk = 0;
+		for( std::int64_t k = 0; k < d; k++ )
+		{
+			tk += f[k];
+			out[off + k] = cn.dst_first + util::gen::position( tk, ttotal, R, std::int32_t( k ) );
+		}
+		off += d;
+	}
+}
+
 // Generates the CSR adjacency of `desc` for `seed` (rows clipped to `width` entries).
 inline csr generate_csr(
     std::vector<util::gen::conn> const & conns, std::int64_t n, std::int64_t width, std::uint64_t seed )
 {
 	csr g;
 	g.row_ptr.assign( n + 1, 0 );
-	// Pass 1: degrees.
-	std::vector<std::int32_t> deg( conns.size() );
-	for( std::int64_t r = 0; r < n; r++ )
-	{
-		std::int64_t total = 0;
-		for( std::uint32_t c = 0; c < conns.size(); c++ )
-		{
-			if( r < conns[c].src_first || r >= conns[c].src_last ) continue;
-			std::int64_t d = util::gen::sample_degree( seed, std::uint32_t( r ), c, conns[c] );
-			if( d > width - total ) d = width - total;
-			total += d;
-		}
-		g.row_ptr[r + 1] = g.row_ptr[r] + total;
-	}
+	for( std::int64_t r = 0; r < n; r++ ) g.row_ptr[r + 1] = g.row_ptr[r] + row_length( conns, r, width, seed );
 	g.col.resize( g.row_ptr[n] );
-	// Pass 2: targets.
-	for( std::int64_t r = 0; r < n; r++ )
-	{
-		std::int64_t off = g.row_ptr[r], total = 0;
-		for( std::uint32_t c = 0; c < conns.size(); c++ )
-		{
-			auto const & cn = conns[c];
-			if( r < cn.src_first || r >= cn.src_last ) continue;
-			std::int64_t d = util::gen::sample_degree( seed, std::uint32_t( r ), c, cn );
-			if( d > width - total ) d = width - total;
-			total += d;
-			std::int32_t const range = cn.dst_last - cn.dst_first;
-			std::int32_t const R = range - std::int32_t( d );
-			// F_0 .. F_d (d + 1 draws), 4 per Philox block.
-			std::vector<std::uint32_t> f( ( d + 1 + 3 ) / 4 * 4 );
-			for( std::uint32_t b = 0; b * 4 < f.size(); b++ )
-				util::gen::exp_fix4( seed, std::uint32_t( r ), c, b, &f[b * 4] );
-			std::uint64_t ttotal = 0;
-			for( std::int64_t k = 0; k <= d; k++ ) ttotal += f[k];
-			std::uint64_t tk = 0;
-			for( std::int64_t k = 0; k < d; k++ )
-			{
-				tk += f[k];
-				g.col[off + k] = cn.dst_first + util::gen::position( tk, ttotal, R, std::int32_t( k ) );
-			}
-			off += d;
-		}
-	}
+	for( std::int64_t r = 0; r < n; r++ ) row_targets( conns, r, width, seed, g.col.data() + g.row_ptr[r] );
+	return g;
+}
+
+// Rows [r0, r1) only (graphs too large to build whole on the host): the absolute offsets
+// row_ptr[r0] .. row_ptr[r1] (every earlier row's length is summed: lengths are cheap, one
+// Philox block per row and connection) and the targets of those rows.
+inline csr generate_rows(
+    std::vector<util::gen::conn> const & conns, std::int64_t r0, std::int64_t r1, std::int64_t width, std::uint64_t seed )
+{
+	csr g;
+	std::int64_t base = 0;
+	for( std::int64_t r = 0; r < r0; r++ ) base += row_length( conns, r, width, seed );
+	g.row_ptr.assign( r1 - r0 + 1, base );
+	for( std::int64_t r = r0; r < r1; r++ ) g.row_ptr[r - r0 + 1] = g.row_ptr[r - r0] + row_length( conns, r, width, seed );
+	g.col.resize( g.row_ptr[r1 - r0] - base );
+	for( std::int64_t r = r0; r < r1; r++ ) row_targets( conns, r, width, seed, g.col.data() + ( g.row_ptr[r - r0] - base ) );
 	return g;
 }
 } // namespace spice::oracle

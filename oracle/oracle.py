@@ -169,6 +169,23 @@ def generate_csr(layout, seed):
     return rp, col[: nnz.value], w.value
 
 
+def generate_rows(layout, seed, row_first, row_last, cols=True):
+    """Host build of rows [row_first, row_last) only: (absolute offsets int64[k + 1], targets
+    int32 of those rows, or None with cols=False)."""
+    pops, conns = _normalize_layout(layout)
+    pa, ca = _layout_args(pops, conns)
+    k = int(row_last) - int(row_first)
+    rp = np.empty(max(k, 0) + 1, np.int64)
+    args = (_ptr(pa, _c.c_uint64), len(pops), ca, len(conns), _c.c_uint64(seed), _c.c_int64(row_first),
+            _c.c_int64(row_last), _c.c_void_p(rp.ctypes.data))
+    _check(lib().oracle_generate_rows(*args, None, _c.c_int64(0)))
+    if not cols:
+        return rp, None
+    col = np.empty(max(int(rp[-1] - rp[0]), 1), np.int32)
+    _check(lib().oracle_generate_rows(*args, _c.c_void_p(col.ctypes.data), _c.c_int64(col.size)))
+    return rp, col[: int(rp[-1] - rp[0])]
+
+
 def degrees(layout, seed):
     """Per (row, flattened connection) clipped degrees; -1 where the connection does not cover the row."""
     pops, conns = _normalize_layout(layout)
@@ -365,6 +382,14 @@ class CpuSnn:
 
     def set_clock(self, step, sum_, comp):
         lib().oracle_set_clock(self._h, _c.c_int64(step), _c.c_float(sum_), _c.c_float(comp))
+
+    def set_in_flight(self, steps):
+        """steps: spike-id lists of the last len(steps) (<= delay) steps, oldest first: the
+        spikes the next steps deliver (replaces the net's own record of them)."""
+        counts = np.array([len(x) for x in steps], np.int64)
+        ids = np.concatenate([np.asarray(x, np.int32) for x in steps] + [np.zeros(1, np.int32)])
+        _check(lib().oracle_set_in_flight(self._h, _c.c_int64(len(steps)), _c.c_void_p(counts.ctypes.data),
+                                          _c.c_void_p(ids.ctypes.data)))
 
 
 class RefSnn:

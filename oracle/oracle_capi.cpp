@@ -122,6 +122,31 @@ int oracle_generate_csr(
 	} );
 }
 
+// Host build of rows [row_first, row_last) only: row_ptr gets the row_last - row_first + 1
+// absolute offsets, col (NULL: offsets only) the targets of those rows (col_cap elements).
+int oracle_generate_rows(
+    uint64_t const * pops,
+    int32_t npops,
+    spice_connection const * conns,
+    int32_t nconns,
+    uint64_t seed,
+    int64_t row_first,
+    int64_t row_last,
+    int64_t * row_ptr,
+    int32_t * col,
+    int64_t col_cap )
+{
+	return guard( [&] {
+		auto l = make_layout( pops, npops, conns, nconns );
+		if( row_first < 0 || row_first > row_last || row_last > int64_t( l.size() ) ) throw std::invalid_argument( "row range out of bounds" );
+		auto g = oracle::generate_rows( oracle::gen_conns( l ), row_first, row_last, int64_t( l.max_degree() ), seed );
+		std::memcpy( row_ptr, g.row_ptr.data(), g.row_ptr.size() * 8 );
+		if( !col ) return;
+		if( int64_t( g.col.size() ) > col_cap ) throw std::invalid_argument( "column buffer too small" );
+		std::memcpy( col, g.col.data(), g.col.size() * 4 );
+	} );
+}
+
 int oracle_create(
     int32_t model,
     uint64_t const * pops,
@@ -284,6 +309,17 @@ int oracle_set_clock( oracle::net_base * net, int64_t i, float sum, float c )
 {
 	net->set_clock( i, sum, c );
 	return SPICE_OK;
+}
+// Spikes in flight: counts[k] ids (concatenated in ids) emitted at step i - nsteps + k, i the
+// next step to run; replaces the net's own record of the last nsteps (<= delay) steps.
+int oracle_set_in_flight( oracle::net_base * net, int64_t nsteps, int64_t const * counts, int32_t const * ids )
+{
+	return guard( [&] {
+		std::vector<int64_t> c( counts, counts + nsteps );
+		int64_t total = 0;
+		for( auto x : c ) total += x;
+		net->set_in_flight( c, std::vector<int>( ids, ids + total ) );
+	} );
 }
 
 // ---- primitives exposed for known-answer and accuracy tests ----

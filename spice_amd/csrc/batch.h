@@ -3,10 +3,10 @@
 // jobs). Every net keeps its own buffers, clock and statistics (it is an ordinary hip_net);
 // the batch adds device tables that map the workgroups of ONE launch per step phase to
 // (net, local tile), (net, local neuron block) and, for plastic models, (net, local flush
-// block), and replays a captured HIP graph of those launches. Per-net results are identical
-// to stepping the nets one by one: the kernels run the same per-net bodies and all randomness
-// is keyed on (net seed, neuron, step). Nets may sit at different steps (served jobs join a
-// running mix): each net's clock slot is chosen from its own step parity.
+// block), and replays captured HIP graphs of those launches (graphs.h). Per-net results are
+// identical to stepping the nets one by one: the kernels run the same per-net bodies and all
+// randomness is keyed on (net seed, neuron, step). Nets may sit at different steps (served
+// jobs join a running mix): each net's clock slot is chosen from its own step parity.
 #include "net.h"
 
 #include <memory>
@@ -28,7 +28,6 @@ struct batch_iface
 template <typename Model>
 class hip_batch final : public batch_iface
 {
-	static constexpr int graph_steps = 32;
 	static constexpr bool plastic = is_plastic_v<Model>;
 	// delivery workgroup size: most tiles of a sweep belong to small nets and carry few events,
 	// so more, smaller workgroups shorten the launch's critical path
@@ -110,30 +109,17 @@ public:
 
 	~hip_batch() override
 	{
-		if( _exec ) (void)hipGraphExecDestroy( _exec );
 		if( _s ) (void)hipStreamSynchronize( _s );
+		_graphs.clear();
 		if( _s && _own_stream ) (void)hipStreamDestroy( _s );
 	}
 
 	void run( std::int64_t n ) override
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
-		std::int64_t done = 0;
-		auto eager = [&] {
-			launch_step( _step++ );
-			done++;
-		};
-		while( ( _step & ( CLOCK_SLOTS - 1 ) ) && done < n ) eager();
-		if( n - done >= graph_steps )
-		{
-			if( !_exec ) capture();
-			for( ; done + graph_steps <= n; done += graph_steps )
-			{
-				SPICE_HIP( hipGraphLaunch( _exec, _s ) );
-				_step += graph_steps;
-			}
-		}
-		while( done < n ) eager();
+		if( n == 0 ) return;
+		if( _graphs.empty() ) _graphs.capture( _s, [&]( int t ) { launch_step( t ); } );
+		_graphs.run( _s, _step, n, [&] { _step++; } );
 		for( auto * net : _nets ) net->advance_host( n );
 	}
 
@@ -160,7 +146,7 @@ private:
 	std::vector<std::unique_ptr<hip_net<Model>>> _owned;
 	hipStream_t _s = nullptr;
 	bool _own_stream = false;
-	hipGraphExec_t _exec = nullptr;
+	step_graphs _graphs;
 	bool _fixed = false;
 	std::int64_t _step = 0;
 	std::int32_t _ntiles = 0, _nblocks = 0, _nflush = 0;
@@ -195,25 +181,6 @@ private:
 		SPICE_LAUNCH( k_batch_update<Model, F><<<_nblocks, 256, 0, _s>>>( v, _block_net.data(), _block_local.data() ) );
 	}
 
-	void capture()
-	{
-		hipGraph_t g = nullptr;
-		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
-		try
-		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step( i );
-		}
-		catch( ... )
-		{
-			(void)hipStreamEndCapture( _s, &g );
-			if( g ) (void)hipGraphDestroy( g );
-			throw;
-		}
-		SPICE_HIP( hipStreamEndCapture( _s, &g ) );
-		hipError_t const e = hipGraphInstantiate( &_exec, g, nullptr, nullptr, 0 );
-		(void)hipGraphDestroy( g );
-		check( e, "hipGraphInstantiate" );
-	}
 };
 
 // Builds the nets of a batch (each gets its own stream for setup) and the batch, which owns

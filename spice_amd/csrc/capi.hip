@@ -96,6 +96,45 @@ static net_params params_of( int32_t model, spice_net_desc const & d, float dt, 
 	return p;
 }
 
+namespace spice::hipx
+{
+// Graph generation into caller-owned device memory (spice_hip_generate_graph*). `scratch` null:
+// the connection table and the scan's tile sums are allocated for the call; otherwise it
+// returns this device's address of a preallocated GRAPH_SCRATCH_BYTES region, whose size
+// limits the connection and neuron counts.
+int generate_graph_entry( void * ( *scratch )(), uint64_t const * pops, int32_t npops, spice_connection const * conns, int32_t nconns,
+                          uint64_t seed, int64_t * d_row_ptr, int32_t * d_col, uint64_t col_cap, uint64_t * out_nnz, uint64_t * out_width )
+{
+	return guard( [&] {
+		require( d_row_ptr != nullptr, "null row offset buffer" );
+		auto l = make_layout( pops, npops, conns, nconns );
+		auto const gc = gen_conns( l );
+		int64_t const n = int64_t( l.size() ), width = int64_t( l.max_degree() );
+		require( n < int64_t( INT_MAX ) - 1024, "too many neurons" );
+		int64_t nnz = 0;
+		if( scratch )
+		{
+			require( gc.size() * sizeof( util::gen::conn ) <= GRAPH_SCRATCH_CONN_BYTES,
+			         "too many connections for the preallocated generator scratch (at most 1024)" );
+			require( std::size_t( graph_scratch_tiles( n ) ) * 8 <= GRAPH_SCRATCH_BYTES - GRAPH_SCRATCH_CONN_BYTES,
+			         "too many neurons for the preallocated generator scratch (at most 130,023,424)" );
+			char * base = static_cast<char *>( scratch() );
+			nnz = generate_into( gc, n, width, seed, d_row_ptr, d_col, col_cap, reinterpret_cast<util::gen::conn *>( base ),
+			                     reinterpret_cast<int64_t *>( base + GRAPH_SCRATCH_CONN_BYTES ), nullptr );
+		}
+		else
+		{
+			dbuf<util::gen::conn> cs( std::max<std::size_t>( gc.size(), 1 ) );
+			dbuf<int64_t> ts( std::size_t( graph_scratch_tiles( n ) ) );
+			nnz = generate_into( gc, n, width, seed, d_row_ptr, d_col, col_cap, cs.data(), ts.data(), nullptr );
+		}
+		if( out_nnz ) *out_nnz = uint64_t( nnz );
+		if( out_width ) *out_width = uint64_t( width );
+		require( d_col == nullptr || uint64_t( nnz ) <= col_cap, "column buffer smaller than the graph (its size is in *out_nnz)" );
+	} );
+}
+} // namespace spice::hipx
+
 extern "C" {
 const char * spice_last_error( void ) { return g_err.c_str(); }
 
@@ -169,6 +208,12 @@ int spice_hip_create(
 		}
 		*out = new spice_net{ make_net( { model, dt, delay, seed, flags, { 0, 0, 0, 0 } }, std::move( g ), s ) };
 	} );
+}
+
+int spice_hip_generate_graph( uint64_t const * pops, int32_t npops, spice_connection const * conns, int32_t nconns, uint64_t seed,
+                              int64_t * d_row_ptr, int32_t * d_col, uint64_t col_cap, uint64_t * out_nnz, uint64_t * out_width )
+{
+	return generate_graph_entry( nullptr, pops, npops, conns, nconns, seed, d_row_ptr, d_col, col_cap, out_nnz, out_width );
 }
 
 int spice_hip_create_desc( int32_t model, spice_net_desc const * d, float dt, uint32_t flags, spice_net ** out )
@@ -289,7 +334,14 @@ int spice_hip_create_from_csr(
 			require( len >= 0 && uint64_t( len ) <= width, "row longer than the adjacency width" );
 		}
 		require( col != nullptr || row_ptr[n] == 0, "null column array" );
-		for( int64_t e = 0; e < row_ptr[n]; e++ ) require( col[e] >= 0 && uint64_t( col[e] ) < n, "neighbor id out of range" );
+		// every row sorted by target: the delivery finds a row's segment inside a target tile by
+		// binary search (the tile offsets), so an unsorted row would put targets outside it
+		for( uint64_t r = 0; r < n; r++ )
+			for( int64_t e = row_ptr[r]; e < row_ptr[r + 1]; e++ )
+			{
+				require( col[e] >= 0 && uint64_t( col[e] ) < n, "neighbor id out of range" );
+				require( e == row_ptr[r] || col[e] >= col[e - 1], "adjacency rows must be sorted by neighbor id" );
+			}
 		require( dt > 0.0f && delay >= 1, "dt must be positive and delay >= 1" );
 		hipStream_t s = new_stream();
 		csr_dev g;
@@ -422,6 +474,15 @@ int spice_hip_export_csr( spice_net * net, int64_t * row_ptr, int32_t * col )
 	return guard( [&] {
 		require( row_ptr != nullptr && ( col != nullptr || impl( net )->num_edges() == 0 ), "null output" );
 		impl( net )->export_csr( row_ptr, col );
+	} );
+}
+int spice_hip_export_csr_rows( spice_net * net, uint64_t row_first, uint64_t row_last, int64_t * row_ptr, int32_t * col, uint64_t col_cap )
+{
+	return guard( [&] {
+		require( row_ptr != nullptr, "null output" );
+		uint64_t const n = uint64_t( impl( net )->num_neurons() );
+		require( row_first <= row_last && row_last <= n, "row range out of bounds" );
+		impl( net )->export_csr_rows( int64_t( row_first ), int64_t( row_last ), row_ptr, col, col_cap );
 	} );
 }
 int spice_hip_export_neuron_attr( spice_net * net, int32_t a, void * out, uint64_t bytes )

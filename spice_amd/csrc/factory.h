@@ -26,6 +26,10 @@ SPICE_MODEL_FACTORIES( brunel_3pop )
 SPICE_MODEL_FACTORIES( brunel_3pop_stdp )
 #undef SPICE_MODEL_FACTORIES
 
+// Graph generation into caller-owned device memory behind spice_hip_generate_graph* (capi.hip).
+int generate_graph_entry( void * ( *scratch )(), uint64_t const * pops, int32_t npops, spice_connection const * conns, int32_t nconns,
+                          uint64_t seed, int64_t * d_row_ptr, int32_t * d_col, uint64_t col_cap, uint64_t * out_nnz, uint64_t * out_width );
+
 inline void require( bool c, char const * msg )
 {
 	if( !c ) throw std::invalid_argument( msg );

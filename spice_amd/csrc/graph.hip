@@ -95,9 +95,10 @@ static __global__ __launch_bounds__( 256 ) void k_targets(
 // ---- exclusive scan int32 -> int64 (3 phases) --------------------------------------------
 constexpr int SCAN_BLOCK = 256, SCAN_ITEMS = 4, SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 
-// Block-level exclusive scan of a tile; writes tile-local prefixes and the tile total.
-static __global__ __launch_bounds__( SCAN_BLOCK ) void k_scan_tiles(
-    std::int32_t const * in, std::int64_t * out, std::int64_t n, std::int64_t * tile_sums )
+// Block-level exclusive scan of a tile; writes tile-local prefixes and the tile total. Each
+// thread reads its items before writing the same positions, so in == out is allowed.
+template <typename In>
+static __global__ __launch_bounds__( SCAN_BLOCK ) void k_scan_tiles( In const * in, std::int64_t * out, std::int64_t n, std::int64_t * tile_sums )
 {
 	__shared__ std::uint64_t wsum[SCAN_BLOCK / 64];
 	std::int64_t const base = std::int64_t( blockIdx.x ) * SCAN_TILE + std::int64_t( threadIdx.x ) * SCAN_ITEMS;
@@ -156,19 +157,28 @@ static __global__ __launch_bounds__( 256 ) void k_scan_add( std::int64_t * out, 
 	if( i < n ) out[i] += tile_sums[i / SCAN_TILE];
 }
 
+// Exclusive scan with the per-tile sums in `sums` (ceil(n / SCAN_TILE) entries, at least 1);
+// out[n] = total. Enqueued on s.
+template <typename In>
+static void exclusive_scan_to_i64( In const * in, std::int64_t * out, std::int64_t n, std::int64_t * sums, hipStream_t s )
+{
+	std::int64_t const ntiles = ( n + SCAN_TILE - 1 ) / SCAN_TILE;
+	if( n > 0 )
+	{
+		SPICE_LAUNCH( k_scan_tiles<In><<<dim3( unsigned( ntiles ) ), SCAN_BLOCK, 0, s>>>( in, out, n, sums ) );
+	}
+	SPICE_LAUNCH( k_scan_tile_sums<<<1, 1024, 0, s>>>( sums, ntiles, out + n ) );
+	if( n > 0 )
+	{
+		SPICE_LAUNCH( k_scan_add<<<dim3( unsigned( ( n + 255 ) / 256 ) ), 256, 0, s>>>( out, n, sums ) );
+	}
+}
+
 void exclusive_scan_i32_to_i64( std::int32_t const * in, std::int64_t * out, std::int64_t n, hipStream_t s )
 {
 	std::int64_t const ntiles = ( n + SCAN_TILE - 1 ) / SCAN_TILE;
 	dbuf<std::int64_t> sums( ntiles > 0 ? ntiles : 1 );
-	if( n > 0 )
-	{
-		SPICE_LAUNCH( k_scan_tiles<<<dim3( unsigned( ntiles ) ), SCAN_BLOCK, 0, s>>>( in, out, n, sums.data() ) );
-	}
-	SPICE_LAUNCH( k_scan_tile_sums<<<1, 1024, 0, s>>>( sums.data(), ntiles, out + n ) );
-	if( n > 0 )
-	{
-		SPICE_LAUNCH( k_scan_add<<<dim3( unsigned( ( n + 255 ) / 256 ) ), 256, 0, s>>>( out, n, sums.data() ) );
-	}
+	exclusive_scan_to_i64( in, out, n, sums.data(), s );
 	SPICE_HIP( hipStreamSynchronize( s ) ); // `sums` is released on return
 }
 
@@ -196,6 +206,49 @@ csr_dev generate_csr( std::vector<conn> const & conns, std::int64_t n, std::int6
 	}
 	SPICE_HIP( hipStreamSynchronize( s ) );
 	return g;
+}
+
+// ---- graph into caller-owned device memory --------------------------------------------------
+// Row lengths go straight into row_ptr[0 .. n) (int64) and are scanned in place, so the only
+// scratch is the connection table and the scan's per-tile sums.
+static __global__ __launch_bounds__( 256 ) void k_degrees_i64(
+    conn const * conns, int nconns, std::int64_t n, std::int64_t width, std::uint64_t seed, std::int64_t * len )
+{
+	std::int64_t const r = std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+	if( r >= n ) return;
+	std::int64_t total = 0;
+	for( int c = 0; c < nconns; c++ )
+	{
+		conn const cn = conns[c];
+		if( r < cn.src_first || r >= cn.src_last ) continue;
+		std::int64_t d = util::gen::sample_degree( seed, std::uint32_t( r ), std::uint32_t( c ), cn );
+		if( d > width - total ) d = width - total;
+		total += d;
+	}
+	len[r] = total;
+}
+
+std::int64_t graph_scratch_tiles( std::int64_t n ) { return n > 0 ? ( n + SCAN_TILE - 1 ) / SCAN_TILE : 1; }
+
+std::int64_t generate_into( std::vector<conn> const & conns, std::int64_t n, std::int64_t width, std::uint64_t seed, std::int64_t * row_ptr,
+                            std::int32_t * col, std::uint64_t col_cap, conn * conn_scratch, std::int64_t * tile_scratch, hipStream_t s )
+{
+	// conn_scratch and tile_scratch need no clearing: the copy below and the scan write every
+	// entry they later read
+	if( !conns.empty() )
+		SPICE_HIP( hipMemcpyAsync( conn_scratch, conns.data(), conns.size() * sizeof( conn ), hipMemcpyHostToDevice, s ) );
+	if( n > 0 )
+		SPICE_LAUNCH( k_degrees_i64<<<dim3( unsigned( ( n + 255 ) / 256 ) ), 256, 0, s>>>( conn_scratch, int( conns.size() ), n, width, seed, row_ptr ) );
+	exclusive_scan_to_i64( row_ptr, row_ptr, n, tile_scratch, s );
+	std::int64_t nnz = 0;
+	SPICE_HIP( hipMemcpyAsync( &nnz, row_ptr + n, sizeof( nnz ), hipMemcpyDeviceToHost, s ) );
+	SPICE_HIP( hipStreamSynchronize( s ) ); // conns (host) copied, nnz read
+	if( col && std::uint64_t( nnz ) <= col_cap && n > 0 && nnz > 0 )
+	{
+		SPICE_LAUNCH( k_targets<<<dim3( unsigned( ( n + 3 ) / 4 ) ), 256, 0, s>>>( conn_scratch, int( conns.size() ), n, width, seed, row_ptr, col ) );
+		SPICE_HIP( hipStreamSynchronize( s ) );
+	}
+	return nnz;
 }
 
 csr_dev upload_csr( std::int64_t const * row_ptr, std::int32_t const * col, std::int64_t n, std::int64_t width, hipStream_t s )

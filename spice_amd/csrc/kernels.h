@@ -389,6 +389,9 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 #ifndef SPICE_UPDATE_BLOCK
 #define SPICE_UPDATE_BLOCK 256 // threads per neuron-update workgroup of a single net (64..256)
 #endif
+// update_block's per-wave compaction counters hold at most 4 waves
+static_assert( SPICE_UPDATE_BLOCK % 64 == 0 && SPICE_UPDATE_BLOCK >= 64 && SPICE_UPDATE_BLOCK <= 256,
+               "SPICE_UPDATE_BLOCK must be 64, 128, 192 or 256 threads" );
 #ifndef SPICE_BATCH_ITEMS_PER_CU
 #define SPICE_BATCH_ITEMS_PER_CU 64
 #endif

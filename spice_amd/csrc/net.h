@@ -1,11 +1,12 @@
 #pragma once
 // Host side of one simulated network on one GPU: owns every device buffer, the HIP stream
-// and the captured step graph, and launches the kernels of kernels.h.
+// and the captured step graphs, and launches the kernels of kernels.h.
 //
 // Reference counterpart: cuda::snn<Model> (spice/cuda/snn.h:16-77, spice/cuda/snn.d.cpp).
 #include "common.h"
 #include "dview.h"
 #include "graph.h"
+#include "graphs.h"
 #include "kernels.h"
 
 #include <spice/models/model.h>
@@ -59,6 +60,8 @@ struct net_iface
 	virtual void get_stats( spice_stats * out ) = 0;
 	virtual void reset_stats() = 0;
 	virtual void export_csr( std::int64_t * row_ptr, std::int32_t * col ) = 0;
+	// rows [r0, r1): r1 - r0 + 1 absolute offsets, and (col != nullptr) their target ids
+	virtual void export_csr_rows( std::int64_t r0, std::int64_t r1, std::int64_t * row_ptr, std::int32_t * col, std::uint64_t col_cap ) = 0;
 	virtual void export_adj( std::int32_t * ell, std::uint64_t cap ) = 0;
 	virtual void export_neuron_attr( int a, void * out, std::uint64_t bytes ) = 0;
 	virtual void import_neuron_attr( int a, void const * in, std::uint64_t bytes ) = 0;
@@ -104,7 +107,6 @@ class hip_net final : public net_iface
 	using S = typename Model::synapse;
 	static constexpr bool plastic = is_plastic_v<Model>;
 	static constexpr bool traced = plastic && has_neuron_traces_v<S>;
-	static constexpr int graph_steps = 32; // steps per captured graph
 
 public:
 	hip_net( net_params const & p, csr_dev && g, hipStream_t stream )
@@ -125,47 +127,29 @@ public:
 
 	~hip_net() override
 	{
-		if( _exec ) (void)hipGraphExecDestroy( _exec );
 		if( _s ) (void)hipStreamSynchronize( _s );
+		_graphs.clear();
 		if( _s ) (void)hipStreamDestroy( _s );
 	}
 
 	// ---- simulation ----
 	void step( std::int32_t ** dspk, std::uint32_t ** dcnt ) override
 	{
-		launch_step( _step );
-		std::int64_t const t = _step++;
-		_clock.add( _p.dt );
+		std::int64_t const t = _step;
+		launch_step( t );
+		tick();
 		int const slot = int( t % ( _p.delay + 1 ) );
 		if( dspk ) *dspk = _ring.data() + std::int64_t( slot ) * _g.n;
 		if( dcnt ) *dcnt = _counts.data() + slot;
 	}
 
+	// Replays captured step graphs (all of them captured at the first run; see graphs.h).
 	void run( std::int64_t n ) override
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
-		std::int64_t done = 0;
-		auto eager = [&] {
-			launch_step( _step );
-			_step++;
-			_clock.add( _p.dt );
-			done++;
-		};
-		while( ( _step & ( CLOCK_SLOTS - 1 ) ) && done < n ) eager(); // graphs start at a step = 0 mod 4
-		if( n - done >= graph_steps )
-		{
-			if( !_exec ) capture();
-			for( ; done + graph_steps <= n; done += graph_steps )
-			{
-				SPICE_HIP( hipGraphLaunch( _exec, _s ) );
-				for( int i = 0; i < graph_steps; i++ )
-				{
-					_step++;
-					_clock.add( _p.dt );
-				}
-			}
-		}
-		while( done < n ) eager();
+		if( n == 0 ) return;
+		if( _graphs.empty() ) _graphs.capture( _s, [&]( int t ) { launch_step( t ); } );
+		_graphs.run( _s, _step, n, [&] { tick(); } );
 	}
 
 	// Runs n steps eagerly, each kernel with HIP events attached to its dispatch on the net's
@@ -190,8 +174,7 @@ public:
 						launch_step_events<true>( e, _vs[_step & ( CLOCK_SLOTS - 1 )] );
 					else
 						launch_step_events<false>( e, _vs[_step & ( CLOCK_SLOTS - 1 )] );
-					_step++;
-					_clock.add( _p.dt );
+					tick();
 				}
 				SPICE_HIP( hipStreamSynchronize( _s ) );
 				for( int i = 0; i < k; i++ )
@@ -277,6 +260,7 @@ public:
 		}
 		_step = step;
 		_clock.set( sum, c );
+		_dt_last = 0.0f;
 	}
 
 	void get_stats( spice_stats * out ) override
@@ -304,6 +288,16 @@ public:
 		synchronize();
 		SPICE_HIP( hipMemcpy( row_ptr, _g.row_ptr.data(), ( _g.n + 1 ) * 8, hipMemcpyDeviceToHost ) );
 		if( _g.nnz ) SPICE_HIP( hipMemcpy( col, _g.col.data(), _g.nnz * 4, hipMemcpyDeviceToHost ) );
+	}
+
+	void export_csr_rows( std::int64_t r0, std::int64_t r1, std::int64_t * row_ptr, std::int32_t * col, std::uint64_t col_cap ) override
+	{
+		synchronize();
+		SPICE_HIP( hipMemcpy( row_ptr, _g.row_ptr.data() + r0, std::size_t( r1 - r0 + 1 ) * 8, hipMemcpyDeviceToHost ) );
+		if( !col ) return;
+		std::int64_t const cnt = row_ptr[r1 - r0] - row_ptr[0];
+		if( std::uint64_t( cnt ) > col_cap ) throw std::invalid_argument( "column buffer too small for the row range" );
+		if( cnt ) SPICE_HIP( hipMemcpy( col, _g.col.data() + row_ptr[0], std::size_t( cnt ) * 4, hipMemcpyDeviceToHost ) );
 	}
 
 	void export_adj( std::int32_t * ell, std::uint64_t cap ) override
@@ -381,7 +375,7 @@ public:
 	// ---- spike log ----
 	void enable_spike_log( std::uint64_t cap, std::int64_t max_steps ) override
 	{
-		if( _exec || _slog_ctl.size() ) throw std::logic_error( "the spike log is enabled once, before the first run" );
+		if( !_graphs.empty() || _slog_ctl.size() ) throw std::logic_error( "the spike log is enabled once, before the first run" );
 		if( max_steps < 1 ) throw std::invalid_argument( "spike log window must hold at least one step" );
 		_slog_ctl.resize( 1 );
 		_slog_ids.resize( std::max<std::uint64_t>( cap, 1 ) );
@@ -505,18 +499,14 @@ public:
 	std::int64_t host_step() const override { return _step; }
 	void advance_host( std::int64_t n )
 	{
-		for( std::int64_t k = 0; k < n; k++ )
-		{
-			_step++;
-			_clock.add( _p.dt );
-		}
+		for( std::int64_t k = 0; k < n; k++ ) tick();
 	}
 
 private:
 	net_params _p;
 	csr_dev _g;
 	hipStream_t _s = nullptr;
-	hipGraphExec_t _exec = nullptr;
+	step_graphs _graphs;
 
 	dbuf<char> _na[MAXA], _acc[MAXA], _sa[MAXA];
 	dbuf<std::int32_t> _ring;
@@ -549,6 +539,14 @@ private:
 	dbuf<std::int32_t> _pivots;
 	std::int64_t _step = 0;
 	util::kahan_sum<float> _clock;
+	float _dt_last = 0.0f; // compensated dt of the last step (what its kernels used)
+
+	// the host's copy of the clock advances with every enqueued step, as the device's does
+	void tick()
+	{
+		_step++;
+		_dt_last = _clock.add( _p.dt );
+	}
 
 	dbuf<slog_t> _slog_ctl;
 	dbuf<std::int32_t> _slog_ids;
@@ -826,12 +824,8 @@ private:
 			_vs[p].ctl_prev = _ctl.data() + ( ( p + CLOCK_SLOTS - 1 ) & ( CLOCK_SLOTS - 1 ) );
 		}
 		if( retile ) SPICE_LAUNCH( k_pivots<<<unsigned( ( n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
-		if( _exec )
-		{
-			(void)hipGraphExecDestroy( _exec ); // the captured launches hold the old shapes
-			_exec = nullptr;
-		}
 		SPICE_HIP( hipStreamSynchronize( _s ) );
+		_graphs.clear(); // the captured launches hold the old shapes
 	}
 
 	void initialize()
@@ -899,26 +893,6 @@ private:
 		launch_timed( reinterpret_cast<void const *>( &k_update<Model, F> ), unsigned( _grid_upd ), SPICE_UPDATE_BLOCK, e[2], e[3], v );
 	}
 
-	void capture()
-	{
-		hipGraph_t g = nullptr;
-		SPICE_HIP( hipStreamBeginCapture( _s, hipStreamCaptureModeThreadLocal ) );
-		try
-		{
-			for( int i = 0; i < graph_steps; i++ ) launch_step( i ); // clock slots 0, 1, 2, 3, 0, ...
-		}
-		catch( ... )
-		{
-			(void)hipStreamEndCapture( _s, &g );
-			if( g ) (void)hipGraphDestroy( g );
-			throw;
-		}
-		SPICE_HIP( hipStreamEndCapture( _s, &g ) );
-		hipError_t const e = hipGraphInstantiate( &_exec, g, nullptr, nullptr, 0 );
-		(void)hipGraphDestroy( g );
-		check( e, "hipGraphInstantiate" );
-	}
-
 	// Brings all plastic synapses up to date through the last completed step.
 	void sync_synapses()
 	{
@@ -958,12 +932,8 @@ private:
 		synchronize();
 	}
 
-	float _last_dt() const
-	{
-		// Recompute the compensated dt of step _step - 1 from the Kahan state after it:
-		// y = sum_after - sum_before - (c_after - c_before) is not recoverable exactly, so
-		// use the nominal dt (the lazy updates of one step differ from it in the last bit).
-		return _p.dt;
-	}
+	// Compensated dt of step _step - 1 (the last enqueued step), as its kernels computed it
+	// from the device clock; the nominal dt before any step (no replay needs it then).
+	float _last_dt() const { return _step > 0 && _dt_last > 0.0f ? _dt_last : _p.dt; }
 };
 } // namespace spice::hipx

@@ -68,6 +68,22 @@ class Layout:
         return pops_a, conns_a
 
 
+def generate_graph(layout, seed, d_row_ptr, d_col=None, col_cap=0):
+    """Random graph of `layout` (the generator of Snn, same seed -> same graph) written to
+    caller-owned device memory: d_row_ptr / d_col are device addresses (ints) of int64[N + 1]
+    and int32[col_cap] buffers (d_col None: offsets only). Returns (nnz, ELL width); raises
+    ValueError if d_col is given and holds fewer than nnz entries."""
+    if not isinstance(layout, Layout):
+        layout = Layout(*layout) if isinstance(layout, tuple) else Layout(layout)
+    pops_a, conns_a = layout._c_args()
+    nnz, width = _c.c_uint64(), _c.c_uint64()
+    _lib.check(_lib.lib().spice_hip_generate_graph(_u64p(pops_a), len(layout.pops), conns_a, len(layout.conns),
+                                                   int(seed), _c.c_void_p(int(d_row_ptr)),
+                                                   _c.c_void_p(int(d_col)) if d_col else None, int(col_cap),
+                                                   _c.byref(nnz), _c.byref(width)))
+    return nnz.value, width.value
+
+
 def _desc(layout, delay, seed, params, keep):
     """NetDesc for a layout; `keep` collects the ctypes buffers the descriptor points to."""
     if not isinstance(layout, Layout):
@@ -290,6 +306,21 @@ class Snn:
         _lib.check(_lib.lib().spice_hip_export_csr(self._h, rp.ctypes.data_as(_c.POINTER(_c.c_int64)),
                                                    col.ctypes.data_as(_c.POINTER(_c.c_int32))))
         return rp, col[:e]
+
+    def csr_rows(self, row_first, row_last, cols=True):
+        """Rows [row_first, row_last) of the CSR adjacency: (absolute offsets int64[k + 1],
+        target ids int32 of those rows, or None with cols=False)."""
+        L = _lib.lib()
+        k = int(row_last) - int(row_first)
+        rp = np.empty(max(k, 0) + 1, dtype=np.int64)
+        pi64 = rp.ctypes.data_as(_c.POINTER(_c.c_int64))
+        _lib.check(L.spice_hip_export_csr_rows(self._h, int(row_first), int(row_last), pi64, None, 0))
+        if not cols:
+            return rp, None
+        col = np.empty(max(int(rp[-1] - rp[0]), 1), dtype=np.int32)
+        _lib.check(L.spice_hip_export_csr_rows(self._h, int(row_first), int(row_last), pi64,
+                                               col.ctypes.data_as(_c.POINTER(_c.c_int32)), col.size))
+        return rp, col[: int(rp[-1] - rp[0])]
 
     def neurons(self):
         """Neuron state as {attribute name: array[N]}."""

@@ -51,3 +51,19 @@ def test_batched_sweep():
     rates = [n[5][0] for n in nets if n[0] == "brunel_3pop"]
     assert min(rates) >= 5 and max(rates) <= 30
     assert len({n[4] for n in nets}) == 512  # per-net seeds
+
+
+def test_reference_synth_networks():
+    """samples/main.cpp:1285-1291: synth is one population, N = sqrt(S / p); the simulation row
+    at p = 0.00156 (delay 1), the setup-time row at p = 0.05."""
+    for name, p in (("ref_synth_1e9", 0.00156), ("ref_synth_setup_1e9", 0.05)):
+        model, n, pp, delay = bench.WORKLOADS[name]
+        assert model == "synth" and pp == p and delay == 1 and n == int(math.sqrt(1e9 / p))
+        pops, conns = bench.workload_layout(name)
+        assert pops == [n] and conns == [(0, 0, p)]
+    assert bench.PUBLISHED["ref_synth_1e9"] == 1.39947 and bench.PUBLISHED_SETUP["ref_synth_setup_1e9"] == 0.253356
+
+
+def test_statistics_populations():
+    assert bench.stat_pops("vogels_abbott", [70_710]) == [56_568, 14_142]
+    assert bench.stat_pops("brunel_3pop", [20, 64, 16]) == [20, 64, 16]

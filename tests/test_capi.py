@@ -69,3 +69,16 @@ def _gpu_present():
 def test_no_silent_cpu_fallback_without_gpu():
     with pytest.raises((RuntimeError, MemoryError)):
         Snn("brunel", Layout(100, 0.1), 1e-4)
+
+
+def test_unsorted_adjacency_rows_are_rejected():
+    """The delivery locates a row's targets inside a tile by binary search, so imported rows
+    must be sorted; an unsorted row is an invalid argument (no GPU work is started)."""
+    rp = np.array([0, 3, 5, 5], np.int64)
+    col = np.array([0, 2, 1, 1, 2], np.int32)  # row 0 unsorted
+    with pytest.raises(ValueError, match="sorted"):
+        Snn.from_csr("brunel", rp, col, 32, 1e-4)
+    ell = np.full((3, 32), np.iinfo(np.int32).max, np.int32)
+    ell[1, :2] = [2, 0]  # row 1 unsorted
+    with pytest.raises(ValueError, match="sorted"):
+        Snn.from_ell("brunel", ell.ravel(), 32, 1e-4)

@@ -126,3 +126,18 @@ def test_target_distribution_matches_reference_generator():
     h2 = np.bincount(ours // 50, minlength=20)
     chi2, pval, _, _ = stats.chi2_contingency(np.vstack([h1, h2]))
     assert pval > 1e-3, (chi2, pval)
+
+
+def test_rows_only_generation_equals_the_whole_graph():
+    """The host build of a row range (used where the whole graph is too large for the host,
+    e.g. 1e10 synapses) equals the same rows of the whole graph: absolute offsets and targets."""
+    lay = ([200, 640, 160], [(s, d, 0.1) for s in range(3) for d in range(3)])
+    rp, col, _ = O.generate_csr(lay, 5)
+    for r0, r1 in [(0, 1000), (0, 0), (17, 18), (333, 777), (999, 1000), (1000, 1000)]:
+        a, b = O.generate_rows(lay, 5, r0, r1)
+        np.testing.assert_array_equal(a, rp[r0:r1 + 1])
+        np.testing.assert_array_equal(b, col[rp[r0]:rp[r1]])
+    a, b = O.generate_rows(lay, 5, 10, 20, cols=False)
+    assert b is None and np.array_equal(a, rp[10:21])
+    with pytest.raises(ValueError):
+        O.generate_rows(lay, 5, 5, 1001)

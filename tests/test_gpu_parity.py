@@ -192,6 +192,28 @@ def test_run_graph_equals_step(gpu):
     assert a.stats() == b.stats()
 
 
+@pytest.mark.parametrize("model", ["brunel_3pop", "brunel_3pop_stdp", "brunel_with_plasticity"])
+def test_short_and_unaligned_runs_replay_graphs(gpu, model):
+    """Runs of any length, from any step, replay captured graphs (1-step graphs per clock
+    phase, 4/8/16/32-step blocks) and equal the same number of spice_hip_step calls, fixed
+    point: run(20) from step 0, then lengths that start and end off the 4-step phase, then a
+    run of 100 steps (more launches than are kept in flight)."""
+    lay = three_pop_layout(2000) if "3pop" in model else brunel_ref_layout(2000)
+    a = Snn(model, Layout(*lay), 1e-4, 15, seed=17, fixed_point=True)
+    b = Snn(model, Layout(*lay), 1e-4, 15, seed=17, fixed_point=True)
+    total = 0
+    for n in (20, 1, 3, 5, 7, 2, 17, 33, 1, 100, 4):
+        b.run(n)
+        for _ in range(n):
+            a.step()
+        total += n
+        assert a.clock() == b.clock() and b.clock()[0] == total
+        assert_state_equal(b.neurons(), a.neurons(), f"after run({n}), step {total}")
+        assert np.array_equal(np.sort(a.step_spikes_of_last()), np.sort(b.step_spikes_of_last())), total
+    assert_state_equal(b.synapses(ell=False), a.synapses(ell=False), "synapses")
+    assert a.stats() == b.stats()
+
+
 @pytest.mark.parametrize("launch", [(256, 128, 1024, 256, 1), (256, 128, 1024, 512, 2), (256, 128, 1024, 1024, 1)],
                          ids=["block256", "block512", "block1024"])
 def test_run_profiled_equals_run(gpu, launch):
@@ -315,3 +337,27 @@ def test_lif_closed_form_on_gpu(gpu, model, fixed):
 
     for delay in (1, 15):
         check_lif_closed_form(make, model, delay).close()
+
+
+@pytest.mark.parametrize("name,layout,seed", GRAPH_CASES[2:6], ids=[c[0] for c in GRAPH_CASES[2:6]])
+def test_generate_graph_into_device_buffers(gpu, name, layout, seed):
+    """spice_hip_generate_graph writes the same graph as the host build into caller-owned
+    device memory (torch tensors here): offsets alone first, then offsets and targets; a
+    column buffer that is too small is rejected with the graph's size reported."""
+    import torch
+    from spice_amd.snn import generate_graph
+    rp_h, col_h, w_h = O.generate_csr(layout, seed)
+    n = rp_h.size - 1
+    rp = torch.full((n + 1,), -1, dtype=torch.int64, device="cuda")
+    nnz, w = generate_graph(layout, seed, rp.data_ptr())
+    assert nnz == col_h.size and w == w_h
+    np.testing.assert_array_equal(rp.cpu().numpy(), rp_h)
+    col = torch.full((max(nnz, 1),), -1, dtype=torch.int32, device="cuda")
+    rp.fill_(-1)
+    assert generate_graph(layout, seed, rp.data_ptr(), col.data_ptr(), col.numel()) == (nnz, w)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(rp.cpu().numpy(), rp_h)
+    np.testing.assert_array_equal(col.cpu().numpy()[:nnz], col_h)
+    if nnz > 1:
+        with pytest.raises(ValueError, match="column buffer"):
+            generate_graph(layout, seed, rp.data_ptr(), col.data_ptr(), nnz - 1)

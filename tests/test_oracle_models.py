@@ -165,3 +165,26 @@ def test_stdp_weight_change_single_pair_closed_form():
     assert np.all((expect > 0) & (expect < 3e-4))
     np.testing.assert_allclose(w[plastic], expect, rtol=1e-5, atol=1e-10)
     assert np.any(pot > 0) and np.any(dep > 0)
+
+
+@pytest.mark.parametrize("model,fixed", [("brunel_3pop", True), ("brunel_3pop", False), ("vogels_abbott", True)])
+def test_checker_resumes_from_state_clock_and_spikes_in_flight(model, fixed):
+    """A fresh checker given another's neuron state, clock and the spikes of the last `delay`
+    steps continues exactly as the original for the next steps (what the full-size GPU tests
+    use to check single steps of a network too large to simulate on the CPU from the start)."""
+    lay = ([200, 640, 160], [(s, d, 0.1) for s in range(3) for d in range(3)]) if model == "brunel_3pop" \
+        else ([1000], [(0, 0, 0.1)])
+    D = 15 if model == "brunel_3pop" else 1
+    h = O.CpuSnn(model, lay, DT, D, seed=3, fixed_point=fixed)
+    sp = [h.step(True) for _ in range(300)]
+    assert sum(len(x) for x in sp[200:]) > 50
+    g = O.CpuSnn(model, lay, DT, D, seed=3, fixed_point=fixed)
+    g.set_neurons(h.neurons())
+    g.set_clock(*h.clock())
+    g.set_in_flight(sp[-D:])
+    for t in range(3 * D):
+        np.testing.assert_array_equal(np.sort(g.step(True)), np.sort(h.step(True)), err_msg=f"step {t}")
+    for k, v in h.neurons().items():
+        np.testing.assert_array_equal(g.neurons()[k], v, err_msg=k)
+    with pytest.raises(ValueError):
+        g.set_in_flight(sp[-D - 1:])  # more than `delay` steps
