@@ -125,6 +125,19 @@ def measured_traffic(workload, kernel):
         return None, None
 
 
+def measured_atomics(workload, kernel):
+    """Atomic operations per launch of `kernel` on `workload` from the committed rocprofv3 PMC
+    summary: LDS atomic lane operations (SQ_INSTS_LDS_ATOMIC_BANDWIDTH, counted in 64-byte units
+    of atomic data, x 16 for 4-byte operands) and L2 atomic requests (TCC_ATOMIC_sum), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)[workload]["kernels"][kernel]
+        return {"lds_atomic_ops": int(t["lds_atomic_ops"]), "l2_atomic_requests": int(t["l2_atomic_requests"])}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def sweep_nets(count, seed):
     """Parameter sweep of the batched configuration: `count` independent nets, alternately
     Brunel (brunel_3pop, p = 0.1, delay 15, Poisson rate ~ U[5, 30] Hz) and Vogels-Abbott
@@ -523,6 +536,16 @@ def main():
         # delivery accumulates every delivered event with one LDS atomic add per receive target
         "receive_lds_atomics_per_s": round(pst["events"] / (ms["receive"] * 1e-3), 1) if ms.get("receive") else None,
     }
+    # the same throughput from the PMC counters of a rocprofv3 run of this workload: atomic
+    # operations counted per launch / this run's mean launch duration
+    counted = measured_atomics(args.workload, kernel_names[kern])
+    if counted is not None:
+        roofline["atomics_counted"] = {
+            "lds_atomic_ops_per_launch": counted["lds_atomic_ops"],
+            "lds_atomic_ops_per_s": round(counted["lds_atomic_ops"] / (avg_ms * 1e-3), 1),
+            "l2_atomic_requests_per_launch": counted["l2_atomic_requests"],
+            "l2_atomic_requests_per_s": round(counted["l2_atomic_requests"] / (avg_ms * 1e-3), 1),
+            "source": "profiles/pmc_traffic.json"}
 
     cpu = None
     if args.cpu_steps > 0:

@@ -103,13 +103,14 @@ inline csr generate_csr(
 // row_ptr[r0] .. row_ptr[r1] (every earlier row's length is summed: lengths are cheap, one
 // Philox block per row and connection) and the targets of those rows.
 inline csr generate_rows(
-    std::vector<util::gen::conn> const & conns, std::int64_t r0, std::int64_t r1, std::int64_t width, std::uint64_t seed )
+    std::vector<util::gen::conn> const & conns, std::int64_t r0, std::int64_t r1, std::int64_t width, std::uint64_t seed, bool cols = true )
 {
 	csr g;
 	std::int64_t base = 0;
 	for( std::int64_t r = 0; r < r0; r++ ) base += row_length( conns, r, width, seed );
 	g.row_ptr.assign( r1 - r0 + 1, base );
 	for( std::int64_t r = r0; r < r1; r++ ) g.row_ptr[r - r0 + 1] = g.row_ptr[r - r0] + row_length( conns, r, width, seed );
+	if( !cols ) return g; // offsets only: no targets are drawn
 	g.col.resize( g.row_ptr[r1 - r0] - base );
 	for( std::int64_t r = r0; r < r1; r++ ) row_targets( conns, r, width, seed, g.col.data() + ( g.row_ptr[r - r0] - base ) );
 	return g;

@@ -139,7 +139,7 @@ int oracle_generate_rows(
 	return guard( [&] {
 		auto l = make_layout( pops, npops, conns, nconns );
 		if( row_first < 0 || row_first > row_last || row_last > int64_t( l.size() ) ) throw std::invalid_argument( "row range out of bounds" );
-		auto g = oracle::generate_rows( oracle::gen_conns( l ), row_first, row_last, int64_t( l.max_degree() ), seed );
+		auto g = oracle::generate_rows( oracle::gen_conns( l ), row_first, row_last, int64_t( l.max_degree() ), seed, col != nullptr );
 		std::memcpy( row_ptr, g.row_ptr.data(), g.row_ptr.size() * 8 );
 		if( !col ) return;
 		if( int64_t( g.col.size() ) > col_cap ) throw std::invalid_argument( "column buffer too small" );

@@ -259,6 +259,8 @@ def test_sweep_512_nets_batched_equals_alone(gpu):
         assert_state_equal(net.neurons(), nrn, f"net {k} ({m})")
         spikes += st["spikes"]
         net.close()
+        if k % 64 == 63:
+            _say(f"sweep: nets 0..{k} equal alone")
     assert spikes > 1_000_000, spikes
     _say(f"sweep: 512 nets equal, {spikes} spikes")
 
@@ -292,4 +294,5 @@ def test_served_stream_with_a_1e9_job(gpu):
         assert rec["digest"] == spike_digest(enumerate(ref)), (k, m)
         assert rec["spikes"] == st["spikes"] and rec["events"] == st["events"], (k, rec, st)
         assert rec["plastic_events"] == st["plastic_events"], k
+        _say(f"serve: job {k} ({m}, {steps} steps) equals the job alone")
     _say(f"serve: {len(jobs)} jobs equal alone; the large job's latency {recs[0]['t_done'] - recs[0]['t_submit']:.2f}s")
