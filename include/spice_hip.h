@@ -216,6 +216,27 @@ int spice_hip_generate_graph(
     uint64_t * out_nnz,
     uint64_t * out_width );
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * results and errors as spice_hip_generate_graph, but the generator's scratch is a fixed
+ * 1 MiB region in the library's own device memory (one per GPU, held until the process ends)
+ * instead of memory allocated for the call; the call allocates no device memory. Limits set
+ * by that region: at most 1024 flattened connections and 130,023,424 neurons (larger inputs
+ * fail with SPICE_ERR_INVALID_ARGUMENT). All calls on one GPU share the region: calls that
+ * overlap on one GPU (from several host threads) race and must be serialised by the caller.
+ * Synchronous; runs on the null stream. */
+int spice_hip_generate_graph_noalloc(
+    const uint64_t * pops,
+    int32_t npops,
+    const spice_connection * conns,
+    int32_t nconns,
+    uint64_t seed,
+    int64_t * d_row_ptr,
+    int32_t * d_col,
+    uint64_t col_cap,
+    uint64_t * out_nnz,
+    uint64_t * out_width );
+
 /* Releases every device buffer of the net (the reference's dbuffer RAII,
  * spice/cuda/util/dbuffer.h:17-90). */
 int spice_hip_destroy( spice_net * net );

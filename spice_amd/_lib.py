@@ -132,6 +132,7 @@ SIGNATURES = {
     "spice_hip_export_adj": (ctypes.c_int, [_P, _PI32, _U64]),
     "spice_hip_export_csr": (ctypes.c_int, [_P, _PI64, _PI32]),
     "spice_hip_generate_graph": (ctypes.c_int, [_PU64, _I32, ctypes.POINTER(Connection), _I32, _U64, _P, _P, _U64, _PU64, _PU64]),
+    "spice_hip_generate_graph_noalloc": (ctypes.c_int, [_PU64, _I32, ctypes.POINTER(Connection), _I32, _U64, _P, _P, _U64, _PU64, _PU64]),
     "spice_hip_export_csr_rows": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _PI64, _PI32, ctypes.c_uint64]),
     "spice_hip_export_neuron_attr": (ctypes.c_int, [_P, _I32, _P, _U64]),
     "spice_hip_import_neuron_attr": (ctypes.c_int, [_P, _I32, _P, _U64]),

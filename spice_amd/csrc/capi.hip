@@ -96,6 +96,33 @@ static net_params params_of( int32_t model, spice_net_desc const & d, float dt, 
 	return p;
 }
 
+// Generator scratch of spice_hip_generate_graph_noalloc: GRAPH_SCRATCH_BYTES in the code
+// object's global segment, so every GPU the library runs on holds its own copy until the
+// process ends. In host code `scratch` is a stand-in for the symbol, not a device address:
+// kernels and copies get the per-device address that graph_scratch() looked up.
+static __device__ __align__( 256 ) char scratch[GRAPH_SCRATCH_BYTES];
+
+// This device's address of `scratch`, looked up once per device (with that device current,
+// at its first no-allocation call) and kept; a fixed table, so the lookup allocates nothing.
+static void * graph_scratch()
+{
+	constexpr int MAX_DEVICES = 64;
+	static std::mutex m;
+	static void * addr[MAX_DEVICES] = {};
+	int dev = 0;
+	SPICE_HIP( hipGetDevice( &dev ) );
+	require( dev >= 0 && dev < MAX_DEVICES, "device id beyond the scratch table" );
+	std::lock_guard<std::mutex> lk( m );
+	if( !addr[dev] )
+	{
+		void * p = nullptr;
+		SPICE_HIP( hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) ) );
+		require( p != nullptr, "hipGetSymbolAddress returned no address for the generator scratch" );
+		addr[dev] = p;
+	}
+	return addr[dev];
+}
+
 namespace spice::hipx
 {
 // Graph generation into caller-owned device memory (spice_hip_generate_graph*). `scratch` null:
@@ -214,6 +241,13 @@ int spice_hip_generate_graph( uint64_t const * pops, int32_t npops, spice_connec
                               int64_t * d_row_ptr, int32_t * d_col, uint64_t col_cap, uint64_t * out_nnz, uint64_t * out_width )
 {
 	return generate_graph_entry( nullptr, pops, npops, conns, nconns, seed, d_row_ptr, d_col, col_cap, out_nnz, out_width );
+}
+
+int spice_hip_generate_graph_noalloc( uint64_t const * pops, int32_t npops, spice_connection const * conns, int32_t nconns,
+                                      uint64_t seed, int64_t * d_row_ptr, int32_t * d_col, uint64_t col_cap, uint64_t * out_nnz,
+                                      uint64_t * out_width )
+{
+	return generate_graph_entry( &graph_scratch, pops, npops, conns, nconns, seed, d_row_ptr, d_col, col_cap, out_nnz, out_width );
 }
 
 int spice_hip_create_desc( int32_t model, spice_net_desc const * d, float dt, uint32_t flags, spice_net ** out )

@@ -68,19 +68,21 @@ class Layout:
         return pops_a, conns_a
 
 
-def generate_graph(layout, seed, d_row_ptr, d_col=None, col_cap=0):
+def generate_graph(layout, seed, d_row_ptr, d_col=None, col_cap=0, noalloc=False):
     """Random graph of `layout` (the generator of Snn, same seed -> same graph) written to
     caller-owned device memory: d_row_ptr / d_col are device addresses (ints) of int64[N + 1]
     and int32[col_cap] buffers (d_col None: offsets only). Returns (nnz, ELL width); raises
-    ValueError if d_col is given and holds fewer than nnz entries."""
+    ValueError if d_col is given and holds fewer than nnz entries. noalloc: the entry point
+    that allocates no device memory (its scratch is the library's fixed 1 MiB region per GPU:
+    at most 1024 flattened connections and 130,023,424 neurons; overlapping calls on one GPU
+    race)."""
     if not isinstance(layout, Layout):
         layout = Layout(*layout) if isinstance(layout, tuple) else Layout(layout)
     pops_a, conns_a = layout._c_args()
     nnz, width = _c.c_uint64(), _c.c_uint64()
-    _lib.check(_lib.lib().spice_hip_generate_graph(_u64p(pops_a), len(layout.pops), conns_a, len(layout.conns),
-                                                   int(seed), _c.c_void_p(int(d_row_ptr)),
-                                                   _c.c_void_p(int(d_col)) if d_col else None, int(col_cap),
-                                                   _c.byref(nnz), _c.byref(width)))
+    fn = _lib.lib().spice_hip_generate_graph_noalloc if noalloc else _lib.lib().spice_hip_generate_graph
+    _lib.check(fn(_u64p(pops_a), len(layout.pops), conns_a, len(layout.conns), int(seed), _c.c_void_p(int(d_row_ptr)),
+                  _c.c_void_p(int(d_col)) if d_col else None, int(col_cap), _c.byref(nnz), _c.byref(width)))
     return nnz.value, width.value
 
 
