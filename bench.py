@@ -414,6 +414,7 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
                "bio U[0.1, 10] s",
         "synapses_total": syn, "events_total": events, "spikes_streamed": streamed[0],
         "makespan_s": round(makespan, 3), "busy_s": round(info["busy_s"], 3),
+        "worker_s": {k: round(info[k], 3) for k in ("launch_s", "admit_s", "wait_s", "collect_s")},
         "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
         "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
         "latency_max_s": round(float(lat.max()), 4),

@@ -366,6 +366,10 @@ typedef struct spice_server_info
 	uint64_t budget, reserved, peak_reserved;
 	int64_t queued, resident, finished, chunks, rebuilds, steps_launched;
 	double busy_s, now;
+	/* the worker's wall time by cycle phase (seconds): enqueueing chunks (batch rebuilds and
+	 * graph captures included), building admitted jobs, waiting for chunks, and collecting
+	 * spikes and retiring jobs (the device has no chunk in flight during the last) */
+	double launch_s, admit_s, wait_s, collect_s;
 } spice_server_info;
 
 /* budget_bytes: device-memory admission budget; chunk_steps: longest chunk (also the spike

@@ -69,7 +69,8 @@ class ServerInfo(ctypes.Structure):
     _fields_ = [("budget", ctypes.c_uint64), ("reserved", ctypes.c_uint64), ("peak_reserved", ctypes.c_uint64),
                 ("queued", ctypes.c_int64), ("resident", ctypes.c_int64), ("finished", ctypes.c_int64),
                 ("chunks", ctypes.c_int64), ("rebuilds", ctypes.c_int64), ("steps_launched", ctypes.c_int64),
-                ("busy_s", ctypes.c_double), ("now", ctypes.c_double)]
+                ("busy_s", ctypes.c_double), ("now", ctypes.c_double), ("launch_s", ctypes.c_double),
+                ("admit_s", ctypes.c_double), ("wait_s", ctypes.c_double), ("collect_s", ctypes.c_double)]
 
 
 SPIKE_SINK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,

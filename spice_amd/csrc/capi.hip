@@ -666,7 +666,9 @@ int spice_server_get_info( spice_server * sv, spice_server_info * out )
 		auto & sc = sched_of( sv );
 		require( out != nullptr, "null output" );
 		serve::server_info const i = sc.info();
-		*out = { i.budget, i.reserved, i.peak_reserved, i.queued, i.resident, i.finished, i.chunks, i.rebuilds, i.steps_launched, i.busy_s, sc.now() };
+		*out = { i.budget,    i.reserved, i.peak_reserved, i.queued,  i.resident, i.finished, i.chunks,
+		         i.rebuilds,  i.steps_launched, i.busy_s,  sc.now(), i.launch_s, i.admit_s,  i.wait_s,
+		         i.collect_s };
 	} );
 }
 

@@ -66,6 +66,10 @@ struct server_info
 	std::int64_t queued = 0, resident = 0, finished = 0, chunks = 0, rebuilds = 0;
 	std::int64_t steps_launched = 0; // sum over chunks of the chunk length
 	double busy_s = 0;                // wall time with a chunk in flight
+	// where the worker's time goes, per cycle phase: enqueueing the chunk (including batch
+	// rebuilds and graph captures), building admitted jobs, waiting for the chunk, and
+	// collecting its spikes and retiring jobs (the device idles in the last)
+	double launch_s = 0, admit_s = 0, wait_s = 0, collect_s = 0;
 };
 
 template <typename Exec>
@@ -349,9 +353,11 @@ private:
 				_ex.run( members, changed, chunk );
 				changed = false;
 			}
+			auto const tl = std::chrono::steady_clock::now();
 			// build newly admitted jobs while the chunk runs
 			std::size_t const nrun = _res.size();
 			if( admit() ) changed = true;
+			auto const ta = std::chrono::steady_clock::now();
 			if( members.empty() )
 			{
 				if( nrun == _res.size() && _res.empty() )
@@ -362,11 +368,15 @@ private:
 				continue;
 			}
 			_ex.wait();
+			auto const tw = std::chrono::steady_clock::now();
 			{
 				std::lock_guard<std::mutex> g( _m );
 				_info.chunks++;
 				_info.steps_launched += chunk;
-				_info.busy_s += std::chrono::duration<double>( std::chrono::steady_clock::now() - tb ).count();
+				_info.busy_s += std::chrono::duration<double>( tw - tb ).count();
+				_info.launch_s += std::chrono::duration<double>( tl - tb ).count();
+				_info.admit_s += std::chrono::duration<double>( ta - tl ).count();
+				_info.wait_s += std::chrono::duration<double>( tw - ta ).count();
 			}
 			// spikes of the chunk, then retirements (only the jobs that ran in it)
 			std::vector<std::size_t> gone;
@@ -431,10 +441,10 @@ private:
 				for( auto it = gone.rbegin(); it != gone.rend(); ++it ) _res.erase( _res.begin() + std::ptrdiff_t( *it ) );
 				changed = true;
 			}
-			if( changed )
 			{
 				std::lock_guard<std::mutex> g( _m );
-				_info.rebuilds++;
+				if( changed ) _info.rebuilds++;
+				_info.collect_s += std::chrono::duration<double>( std::chrono::steady_clock::now() - tw ).count();
 			}
 		}
 	}
