@@ -11,8 +11,10 @@
 //
 // Replay keeps at most INFLIGHT graph launches queued on the stream: before launch i the host
 // waits for launch i - INFLIGHT (an event recorded after each launch). A long run (100,000
-// steps are 3,125 launches of 64 kernels) thus never queues more than 8 x 64 dispatches
-// ahead of the GPU, whatever the caller asks for.
+// steps are 3,125 launches of 64 kernels) thus never queues more than 64 x 64 dispatches
+// ahead of the GPU, whatever the caller asks for, while a chunk of up to ~2,000 steps is
+// enqueued without the host waiting (so callers that drive several streams, like the server's
+// per-model batches, get all their chunks queued at once and running side by side).
 #include "common.h"
 #include "dview.h"
 
@@ -25,7 +27,7 @@ class step_graphs
 public:
 	static constexpr int BLOCKS = 4;   // block graphs of 4, 8, 16 and 32 steps
 	static_assert( BLOCKS == CLOCK_SLOTS, "clear() walks both graph sets with one bound" );
-	static constexpr int INFLIGHT = 8; // graph launches queued ahead of the GPU, at most
+	static constexpr int INFLIGHT = 64; // graph launches queued ahead of the GPU, at most
 
 	step_graphs() = default;
 	step_graphs( step_graphs const & ) = delete;

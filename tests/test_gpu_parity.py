@@ -196,13 +196,13 @@ def test_run_graph_equals_step(gpu):
 def test_short_and_unaligned_runs_replay_graphs(gpu, model):
     """Runs of any length, from any step, replay captured graphs (1-step graphs per clock
     phase, 4/8/16/32-step blocks) and equal the same number of spice_hip_step calls, fixed
-    point: run(20) from step 0, then lengths that start and end off the 4-step phase, then a
-    run of 100 steps (more launches than are kept in flight)."""
+    point: run(20) from step 0, then lengths that start and end off the 4-step phase, then runs
+    of 100 and 2,500 steps (the latter: 79 graph launches, more than the 64 kept in flight)."""
     lay = three_pop_layout(2000) if "3pop" in model else brunel_ref_layout(2000)
     a = Snn(model, Layout(*lay), 1e-4, 15, seed=17, fixed_point=True)
     b = Snn(model, Layout(*lay), 1e-4, 15, seed=17, fixed_point=True)
     total = 0
-    for n in (20, 1, 3, 5, 7, 2, 17, 33, 1, 100, 4):
+    for n in (20, 1, 3, 5, 7, 2, 17, 33, 1, 100, 4, 2500, 3):
         b.run(n)
         for _ in range(n):
             a.step()
