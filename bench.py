@@ -347,16 +347,14 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
     from spice_amd.raster import Raster
     jobs = serve_stream(count, rate, seed)
     sample = list(range(0, count, seq_stride))
-    streamed = [0]
-    rasters = {}  # job id -> Raster of the sampled jobs (rate / ISI CV vs the job run alone)
+    chunks = {}  # job id -> the sampled job's spike chunks, turned into a raster after the run
+    sampled_ids = set()
 
     def sink(job, first, off, ids):
-        streamed[0] += len(ids)
-        r = rasters.get(job)
-        if r is not None:
-            r.add(first, off, ids)
+        # runs on the server's worker thread between chunks: only queue the chunk
+        chunks[job].append((first, off, ids))
 
-    sv = Server(budget_bytes=budget, chunk_steps=chunk, on_spikes=sink)
+    sv = Server(budget_bytes=budget, chunk_steps=chunk, on_spikes=sink, spike_jobs=sampled_ids)
     t0 = time.perf_counter()
     ids = []
     for k, (t, m, pops, conns, d, steps, js) in enumerate(jobs):
@@ -365,10 +363,12 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
             time.sleep(lag)
         lay = Layout(pops, conns)
         if k in sample:
-            # registered before submit: the worker may deliver the first chunk at once
-            rasters[len(ids) + 1] = Raster(sum(pops), pops)
+            # registered before submit (ids count up from 1): the worker may deliver the first
+            # chunk at once
+            chunks[len(ids) + 1] = []
+            sampled_ids.add(len(ids) + 1)
         ids.append(sv.submit(m, lay, 1e-4, steps, delay=d, seed=js))
-        assert k not in sample or ids[-1] in rasters
+        assert k not in sample or ids[-1] in chunks
     recs = [sv.wait(i) for i in ids]
     makespan = time.perf_counter() - t0
     info = sv.info()
@@ -378,6 +378,13 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
     qdelay = np.array([r["t_admit"] - r["t_submit"] for r in recs])  # waiting for admission (budget)
     events = sum(r["events"] for r in recs)
     syn = sum(r["synapses"] for r in recs)
+    streamed = sum(r["spikes"] for r in recs)  # every job's spikes reached host memory (counted there)
+    rasters = {}
+    for k in sample:
+        _, m, pops, conns, d, steps, js = jobs[k]
+        r = rasters[ids[k]] = Raster(sum(pops), pops)
+        for first, off, spk in chunks[ids[k]]:
+            r.add(first, off, spk)
     print(f"[bench] served {count} jobs ({syn:.3g} synapses, {events:.3g} events) in {makespan:.1f}s: "
           f"p50 {np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s, {len(failed)} not done",
           file=sys.stderr, flush=True)
@@ -412,7 +419,7 @@ def run_serving(count, rate, budget, chunk, seq_stride, seed):
         "jobs": count, "arrival_rate_per_s": rate, "budget_bytes": budget, "chunk_steps": chunk,
         "mix": "40% brunel_3pop / 40% vogels_abbott / 20% brunel_3pop_stdp, synapses log-uniform 1e5-2e9, "
                "bio U[0.1, 10] s",
-        "synapses_total": syn, "events_total": events, "spikes_streamed": streamed[0],
+        "synapses_total": syn, "events_total": events, "spikes_streamed": streamed,
         "makespan_s": round(makespan, 3), "busy_s": round(info["busy_s"], 3),
         "worker_s": {k: round(info[k], 3) for k in ("launch_s", "admit_s", "wait_s", "collect_s")},
         "latency_p50_s": round(float(np.percentile(lat, 50)), 4),

@@ -48,11 +48,17 @@ class Server:
     """A spice_server. budget_bytes: admission budget of device memory; chunk_steps: the
     longest chunk of steps between spike deliveries. on_spikes(job, first_step, offsets, ids)
     is called on the library's worker thread after every chunk with numpy copies (offsets has
-    nsteps + 1 entries); keep_spikes=True also keeps every job's stream for spikes(job)."""
+    nsteps + 1 entries); keep_spikes=True also keeps every job's stream for spikes(job).
+    spike_jobs: None (every job), or a set of job ids (the caller may add to it, e.g. the id
+    the next submit will return, as ids count up from 1) whose spikes reach on_spikes and
+    keep_spikes; the other jobs' spikes are still streamed to host memory and digested by the
+    library (spice_job_result.spikes / .digest) but not copied into Python. The callback runs
+    on the server's worker thread, between chunks: work done in it delays the next chunk."""
 
-    def __init__(self, budget_bytes=250e9, chunk_steps=1000, on_spikes=None, keep_spikes=False):
+    def __init__(self, budget_bytes=250e9, chunk_steps=1000, on_spikes=None, keep_spikes=False, spike_jobs=None):
         self._on_spikes = on_spikes
         self._keep = keep_spikes
+        self._spike_jobs = spike_jobs
         self._streams = {}
         self._lock = threading.Lock()
         self._chunk = int(chunk_steps)
@@ -62,6 +68,8 @@ class Server:
 
     def _deliver(self, user, job, first_step, nsteps, offsets, ids):
         if self._on_spikes is None and not self._keep:
+            return
+        if self._spike_jobs is not None and int(job) not in self._spike_jobs:
             return
         off = np.ctypeslib.as_array(offsets, shape=(nsteps + 1,)).copy()
         n = int(off[-1])

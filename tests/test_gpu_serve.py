@@ -78,6 +78,30 @@ def test_each_job_alone_in_a_server_has_the_mix_digest(gpu):
         assert alone["events"] == mixed[k]["events"], k
 
 
+def test_spike_jobs_filter_delivers_only_the_listed_jobs(gpu):
+    """Server(spike_jobs=...): only the listed jobs' spikes reach on_spikes and keep_spikes
+    (an id added before its submit counts); every job's spikes are still counted and digested
+    by the library, equal to the job alone."""
+    jobs = _jobs()[:3]
+    listed = {2}
+    seen = set()
+    with Server(budget_bytes=8e9, chunk_steps=200, keep_spikes=True, spike_jobs=listed,
+                on_spikes=lambda job, first, off, ids: seen.add(job)) as sv:
+        ids = [sv.submit(m, lay, DT, steps, delay=d, seed=s, params=p, fixed_point=True)
+               for m, lay, steps, d, s, p in jobs]
+        recs = [sv.wait(i, timeout=120) for i in ids]
+        streams = [sv.spikes(i) for i in ids]
+    assert ids == [1, 2, 3] and seen == {2}
+    assert len(streams[0]) == 0 and len(streams[2]) == 0 and len(streams[1]) == jobs[1][2]
+    for k, ((m, lay, steps, d, s, p), rec) in enumerate(zip(jobs, recs)):
+        ref, st = _alone(m, lay, steps, d, s, p)
+        assert rec["status"] == "done" and rec["spikes"] == st["spikes"], (k, rec, st)
+        assert rec["digest"] == spike_digest(enumerate(ref)), k
+        if k == 1:
+            for t in range(steps):
+                np.testing.assert_array_equal(np.sort(streams[1][t]), ref[t], err_msg=f"step {t}")
+
+
 def test_fp32_jobs_match_alone_statistically(gpu):
     """fp32 accumulation: sums are ordered differently in every launch shape, so served and
     plain runs agree in their statistics, not spike for spike."""
