@@ -237,6 +237,24 @@ __device__ inline void store_tuple( typename Decl::tuple_t const & s, void * con
 	} );
 }
 
+// Stores the attributes of `after` that differ (bit for bit) from `before`, the values loaded
+// for index i. An event that leaves a synapse unchanged -- a potentiation of a weight clamped at
+// its maximum, a depression of one clamped at zero -- then writes nothing, and a memory line
+// none of whose synapses changed is never dirtied: for the scattered post-pass weights that
+// saves the line's write-back. Equal results: an unchanged value is stored by not storing it.
+template <typename Decl, bool NT = false>
+__device__ inline void store_changed( typename Decl::tuple_t const & before, typename Decl::tuple_t const & after, void * const * arr,
+                                      std::int64_t i )
+{
+	util::for_n<Decl::size>( [&]( auto I ) {
+		using T = typename Decl::template type<I>;
+		using B = std::conditional_t<sizeof( T ) == 8, std::uint64_t, std::uint32_t>;
+		static_assert( sizeof( T ) == sizeof( B ), "synapse attributes are 4 or 8 bytes" );
+		if( __builtin_bit_cast( B, std::get<I>( after ) ) != __builtin_bit_cast( B, std::get<I>( before ) ) )
+			sts<NT>( static_cast<T *>( arr[I] ) + i, std::get<I>( after ) );
+	} );
+}
+
 // ---- spike history and lazy plasticity --------------------------------------------------
 // Spike history (plastic models): bit (t mod HISTORY) of ring word ((t mod HISTORY) / 64) of
 // neuron j is set iff j spiked at step t, valid for the last HISTORY steps. Lazy plasticity
@@ -835,8 +853,9 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 							// the event of step T - 1, keyed like the pass that would have applied it
 							auto sbak = make_backend<FIXED>( v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ),
 							                                 std::uint32_t( T - 1 ), util::TAG_SYNAPSE_UPDATE );
+							typename S::tuple_t const before = sy[u];
 							S::on_event( itr, pre, post, post ? seg_xf[kk[u]] : 0.0f, yv[u], info, sbak );
-							store_tuple<S, ( SPICE_NT & 4 ) != 0>( sy[u], v.sa, e[u] );
+							store_changed<S, ( SPICE_NT & 4 ) != 0>( before, sy[u], v.sa, e[u] );
 							pev++;
 						}
 						N::receive( s, dst, itr, info, bak );
@@ -1297,8 +1316,9 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 					    v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
 					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e ) };
 					bool const post = ( f8[u] & F8_SPIKED ) != 0;
+					typename S::tuple_t const before = sy[u];
 					S::on_event( itr, true, post, post ? xs : 0.0f, yv[u], info, bak );
-					store_tuple<S, NTPRE>( sy[u], v.sa, e );
+					store_changed<S, NTPRE>( before, sy[u], v.sa, e );
 					touched++;
 				}
 			}
@@ -1353,8 +1373,9 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 					auto bak = make_backend<FIXED>(
 					    v, std::uint32_t( e[u] ), std::uint32_t( std::uint64_t( e[u] ) >> 32 ), std::uint32_t( T ), util::TAG_SYNAPSE_UPDATE );
 					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
+					typename S::tuple_t const before = sy[u];
 					S::on_event( itr, false, true, xv[u], 0.0f, info, bak ); // post only: y is not read
-					store_tuple<S, NTPOST>( sy[u], v.sa, e[u] );
+					store_changed<S, NTPOST>( before, sy[u], v.sa, e[u] );
 					touched++;
 				}
 			}
@@ -1524,8 +1545,9 @@ __global__ __launch_bounds__( 256 ) void k_sync_eager( dview v, std::int64_t now
 		load_tuple<S>( sy, v.sa, e );
 		auto bak = make_backend<false>( v, std::uint32_t( e ), std::uint32_t( std::uint64_t( e ) >> 32 ), std::uint32_t( now ), util::TAG_SYNAPSE_UPDATE );
 		reg_iter<typename S::tuple_t> it{ sy, std::uint32_t( e ) };
+		typename S::tuple_t const before = sy;
 		S::on_event( it, pre, post, post ? xs : 0.0f, pre ? v.ytr[prev + j] : 0.0f, info, bak );
-		store_tuple<S>( sy, v.sa, e );
+		store_changed<S>( before, sy, v.sa, e );
 	}
 }
 
