@@ -92,8 +92,10 @@ def algorithmic_bytes(model, n, st, steps):
                index word read (lazy flush: the target id
                of every scanned slot of the flushed rows; eager: the target id of every plastic
                slot of the arriving rows and the (source, offset) pair of every by-target entry
-               of the spiking targets) + per changed synapse its state read + write (8
-               neuron-trace, 24 per-synapse-trace).
+               of the spiking targets) + per synapse brought up to date its state read (4
+               neuron-trace, 12 per-synapse-trace) + per synapse written back its state write
+               (eager passes write only synapses whose state the event changed; the lazy flush
+               writes every one).
     Target-neuron accumulation is counted as atomics, not bytes."""
     from spice_amd.models import MODELS
     _, nattrs, sattrs = MODELS[model]
@@ -108,7 +110,7 @@ def algorithmic_bytes(model, n, st, steps):
     wb = 4 if traced else (2 * s_s - 4)
     recv = spk * 20 + ev * (8 if plastic else 4) + pev * (wb if plastic else 0)
     upd = n * (2 * s_n + 2 * acc_bytes + ((17 + 17) if traced else (25 if plastic else 0))) + spk * 4
-    fl = (st["flush_slots"] * 4 + st["flush_synapses"] * (2 * s_s)) / steps if plastic else 0.0
+    fl = (st["flush_slots"] * 4 + (st["flush_synapses"] + st["flush_written"]) * s_s) / steps if plastic else 0.0
     # the plasticity work of step T - 1 runs inside the delivery kernel of step T
     return {"receive": recv + fl, "update": upd, "flush": 0.0}
 

@@ -114,6 +114,9 @@ typedef struct
 	uint64_t flush_rows;
 	uint64_t flush_synapses;
 	uint64_t flush_slots;
+	/* plastic synapses the passes wrote back: eager, only those whose state the event changed
+	 * (an unchanged synapse is read, not written); lazy, every replayed synapse */
+	uint64_t flush_written;
 } spice_stats;
 
 /* Thread-local message of the last failed call on this thread ("" if none). */

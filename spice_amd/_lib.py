@@ -43,7 +43,8 @@ class NetDesc(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
-                ("steps", "spikes", "events", "plastic_events", "flush_rows", "flush_synapses", "flush_slots")]
+                ("steps", "spikes", "events", "plastic_events", "flush_rows", "flush_synapses", "flush_slots",
+                 "flush_written")]
 
 
 class Launch(ctypes.Structure):

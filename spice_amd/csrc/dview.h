@@ -64,6 +64,7 @@ enum stat_index
 	ST_FLUSH_ROWS,
 	ST_FLUSH_SYN,
 	ST_FLUSH_SLOTS,
+	ST_FLUSH_WRITTEN, // plastic synapses whose state the plasticity passes wrote back
 	NSTAT = 8
 };
 

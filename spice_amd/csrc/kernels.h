@@ -242,17 +242,23 @@ __device__ inline void store_tuple( typename Decl::tuple_t const & s, void * con
 // its maximum, a depression of one clamped at zero -- then writes nothing, and a memory line
 // none of whose synapses changed is never dirtied: for the scattered post-pass weights that
 // saves the line's write-back. Equal results: an unchanged value is stored by not storing it.
+// Returns whether anything was stored.
 template <typename Decl, bool NT = false>
-__device__ inline void store_changed( typename Decl::tuple_t const & before, typename Decl::tuple_t const & after, void * const * arr,
+__device__ inline bool store_changed( typename Decl::tuple_t const & before, typename Decl::tuple_t const & after, void * const * arr,
                                       std::int64_t i )
 {
+	bool any = false;
 	util::for_n<Decl::size>( [&]( auto I ) {
 		using T = typename Decl::template type<I>;
 		using B = std::conditional_t<sizeof( T ) == 8, std::uint64_t, std::uint32_t>;
 		static_assert( sizeof( T ) == sizeof( B ), "synapse attributes are 4 or 8 bytes" );
 		if( __builtin_bit_cast( B, std::get<I>( after ) ) != __builtin_bit_cast( B, std::get<I>( before ) ) )
+		{
 			sts<NT>( static_cast<T *>( arr[I] ) + i, std::get<I>( after ) );
+			any = true;
+		}
 	} );
+	return any;
 }
 
 // ---- spike history and lazy plasticity --------------------------------------------------
@@ -1268,7 +1274,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 	int const lane = lane_id();
 	std::int64_t const wave0 = uniform64( ( blk * blockDim.x + threadIdx.x ) >> 6 );
 	std::int64_t const nwaves = nblk * ( blockDim.x >> 6 );
-	unsigned long long touched = 0, words = 0, rows = 0;
+	unsigned long long touched = 0, written = 0, words = 0, rows = 0;
 	for( std::int64_t it = wave0; it < nitems; it += nwaves )
 	{
 		if( it < npre )
@@ -1318,7 +1324,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 					bool const post = ( f8[u] & F8_SPIKED ) != 0;
 					typename S::tuple_t const before = sy[u];
 					S::on_event( itr, true, post, post ? xs : 0.0f, yv[u], info, bak );
-					store_changed<S, NTPRE>( before, sy[u], v.sa, e );
+					written += store_changed<S, NTPRE>( before, sy[u], v.sa, e ) ? 1 : 0;
 					touched++;
 				}
 			}
@@ -1375,7 +1381,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 					reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
 					typename S::tuple_t const before = sy[u];
 					S::on_event( itr, false, true, xv[u], 0.0f, info, bak ); // post only: y is not read
-					store_changed<S, NTPOST>( before, sy[u], v.sa, e[u] );
+					written += store_changed<S, NTPOST>( before, sy[u], v.sa, e[u] ) ? 1 : 0;
 					touched++;
 				}
 			}
@@ -1386,7 +1392,7 @@ __device__ inline void stdp_block( dview const & v, std::int64_t T, std::int64_t
 			}
 		}
 	}
-	block_add_stats<3>( v, slot0 + blk, { touched, words, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	block_add_stats<4>( v, slot0 + blk, { touched, written, words, rows }, { ST_FLUSH_SYN, ST_FLUSH_WRITTEN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
 // Eager plasticity of step `T` on its own (a synchronization before a state export applies
@@ -1461,7 +1467,7 @@ __device__ inline void flush_block( dview const & v, std::int64_t T, std::int64_
 			if( p == 0 ) rows++;
 		}
 	}
-	block_add_stats<3>( v, slot0 + blk, { touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
+	block_add_stats<4>( v, slot0 + blk, { touched, touched, slots, rows }, { ST_FLUSH_SYN, ST_FLUSH_WRITTEN, ST_FLUSH_SLOTS, ST_FLUSH_ROWS } );
 }
 
 // The lazy flush of step `T` on its own (a synchronization before a state export runs the

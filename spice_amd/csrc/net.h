@@ -272,7 +272,7 @@ public:
 		unsigned long long t[NSTAT] = {};
 		for( std::size_t i = 0; i < b.size(); i++ ) t[i % NSTAT] += b[i];
 		*out = { std::uint64_t( _step - _stats_step0 ), t[ST_SPIKES], t[ST_EVENTS], t[ST_PLASTIC], t[ST_FLUSH_ROWS],
-		         t[ST_FLUSH_SYN], t[ST_FLUSH_SLOTS] };
+		         t[ST_FLUSH_SYN], t[ST_FLUSH_SLOTS], t[ST_FLUSH_WRITTEN] };
 	}
 	void reset_stats() override
 	{

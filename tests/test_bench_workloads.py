@@ -67,3 +67,19 @@ def test_reference_synth_networks():
 def test_statistics_populations():
     assert bench.stat_pops("vogels_abbott", [70_710]) == [56_568, 14_142]
     assert bench.stat_pops("brunel_3pop", [20, 64, 16]) == [20, 64, 16]
+
+
+def test_algorithmic_bytes_model():
+    """The per-launch byte model on hand-made counters (one step): delivery reads 4 B per
+    spike id + 16 B of row offsets per spike + the target id (and, with synapse state, the
+    weight) per event; plasticity reads 4 B per index word, the state of every synapse brought
+    up to date, and writes back only the synapses it changed; static models have no passes."""
+    st = {"spikes": 100, "events": 1000, "plastic_events": 10, "flush_slots": 200, "flush_synapses": 50,
+          "flush_written": 30}
+    b = bench.algorithmic_bytes("brunel_3pop_stdp", 1000, st, 1)
+    assert b["receive"] == 100 * 20 + 1000 * 8 + 10 * 4 + 200 * 4 + (50 + 30) * 4
+    assert b["flush"] == 0.0
+    s = bench.algorithmic_bytes("brunel_3pop", 1000, dict(st, plastic_events=0), 1)
+    assert s["receive"] == 100 * 20 + 1000 * 4
+    # update: state (8 B) read + write, the V accumulator read + clear, 4 B per spike
+    assert s["update"] == 1000 * (2 * 8 + 2 * 4) + 100 * 4
