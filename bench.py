@@ -296,11 +296,14 @@ def run_batched(count, steps, seq_stride):
     alone, _ = _served_rasters(jobs, False, False)
     stats_fp32 = compare_rasters(alone, together, dt, 10_000)
     _, fx_together = _served_rasters(jobs, True, True)
-    _, fx_alone = _served_rasters(jobs, True, False)
+    fx_alone_r, fx_alone = _served_rasters(jobs, True, False)
+    # noise floor: the same nets alone, fp32 against fixed-point accumulation -- only the
+    # rounding of the sums differs, so this is how far chaos alone moves a net's statistics
+    stats_noise = compare_rasters(fx_alone_r, alone, dt, 10_000)
     fx_same = all(fx_together[k]["digest"] == fx_alone[k]["digest"] and fx_together[k]["spikes"] == fx_alone[k]["spikes"]
                   for k in range(len(jobs)))
     print(f"[bench] sweep statistics ({len(jobs)} nets, {time.perf_counter() - t1:.1f}s): {stats_fp32}; "
-          f"fixed point identical: {fx_same}", file=sys.stderr, flush=True)
+          f"noise floor {stats_noise}; fixed point identical: {fx_same}", file=sys.stderr, flush=True)
     return {"nets": count, "synapses": syn, "steps": steps, "bio_s": steps * dt, "setup_s": round(setup, 3),
             "wall_s": round(wall, 4), "syn_events_per_s": round(agg, 1),
             "sequential_syn_events_per_s": round(seq, 1), "speedup_vs_sequential": round(agg / seq, 3),
@@ -312,6 +315,9 @@ def run_batched(count, steps, seq_stride):
             "statistics_sample": f"the same {len(sample)} nets, {steps} steps: stepped together in batched chunks "
                                  f"(server) vs each alone; per net and population",
             "statistics_fp32": stats_fp32,
+            "statistics_noise_floor": stats_noise,
+            "statistics_noise_floor_method": "the same nets alone, fp32 against fixed-point accumulation (rounding "
+                                             "of the sums is all that differs)",
             "fixed_point_identical": fx_same}
 
 
