@@ -42,6 +42,8 @@ WORKLOADS = {
     # name: (model, n, p, delay); brunel* models: three populations (Poisson / E / I), every
     # pair at p; vogels: one population at p
     "brunel_stdp_1e9": ("brunel_3pop_stdp", 100_000, 0.1, 15),
+    # the CPU-runnable configuration: three-population Brunel at 1e6 synapses (N = 3,162)
+    "brunel_1e6": ("brunel_3pop", 3_162, 0.1, 15),
     "brunel_1e9": ("brunel_3pop", 100_000, 0.1, 15),
     "vogels_1e8": ("vogels_abbott", 70_710, 0.02, 1),
     "brunel_1e10": ("brunel_3pop", 316_228, 0.1, 15),

@@ -83,3 +83,11 @@ def test_algorithmic_bytes_model():
     assert s["receive"] == 100 * 20 + 1000 * 4
     # update: state (8 B) read + write, the V accumulator read + clear, 4 B per spike
     assert s["update"] == 1000 * (2 * 8 + 2 * 4) + 100 * 4
+
+
+def test_cpu_runnable_configuration():
+    """brunel_1e6: three populations (Poisson 20 %, E, I) of N = 3,162, all pairs at p = 0.1,
+    ~1e6 synapses."""
+    pops, conns = bench.workload_layout("brunel_1e6")
+    assert sum(pops) == 3_162 and pops[0] == 632 and len(conns) == 9
+    assert abs(_synapses(pops, conns) - 1e6) / 1e6 < 1e-3
