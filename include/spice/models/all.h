@@ -39,6 +39,10 @@ template <>
 struct model_id<brunel_3pop_stdp> : std::integral_constant<int, SPICE_MODEL_BRUNEL_3POP_STDP>
 {
 };
+template <>
+struct model_id<brunel_3pop_stdp_deferred> : std::integral_constant<int, SPICE_MODEL_BRUNEL_3POP_STDP_DEFERRED>
+{
+};
 
 // Calls f(Model{}) for the model with the given id; throws std::invalid_argument otherwise.
 template <typename F>
@@ -52,6 +56,7 @@ decltype( auto ) dispatch_model( int id, F && f )
 	case SPICE_MODEL_SYNTH: return f( synth{} );
 	case SPICE_MODEL_BRUNEL_3POP: return f( brunel_3pop{} );
 	case SPICE_MODEL_BRUNEL_3POP_STDP: return f( brunel_3pop_stdp{} );
+	case SPICE_MODEL_BRUNEL_3POP_STDP_DEFERRED: return f( brunel_3pop_stdp_deferred{} );
 	default: throw std::invalid_argument( "unknown model id" );
 	}
 }

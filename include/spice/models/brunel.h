@@ -56,17 +56,29 @@ struct brunel : model
 			return false;
 		}
 
-		template <typename Iter, typename SynIter, typename Backend>
-		HYBRID static void receive( int_ src, Iter dst, SynIter &&, snn_info info, Backend & bak )
+		// receive() = accepts() ? deliver() : nothing (split so that the GPU may test the gate
+		// in the neuron update)
+		static constexpr bool gated_receive = true;
+
+		template <typename Iter>
+		HYBRID static bool accepts( Iter n )
 		{
-			using util::get;
-			if( get<Twait>( dst ) <= 0 )
-			{
-				int_ const nexc = static_cast<int_>( 0.9f * info.num_neurons );
-				float const Wex = 0.0001f * 20'000 / info.num_neurons;  // V
-				float const Win = -0.0005f * 20'000 / info.num_neurons; // V
-				bak.atomic_add( get<V>( dst ), src < nexc ? Wex : Win );
-			}
+			return util::get<Twait>( n ) <= 0;
+		}
+
+		template <typename Iter, typename SynIter, typename Backend>
+		HYBRID static void deliver( int_ src, Iter dst, SynIter &&, snn_info info, Backend & bak )
+		{
+			int_ const nexc = static_cast<int_>( 0.9f * info.num_neurons );
+			float const Wex = 0.0001f * 20'000 / info.num_neurons;  // V
+			float const Win = -0.0005f * 20'000 / info.num_neurons; // V
+			bak.atomic_add( util::get<V>( dst ), src < nexc ? Wex : Win );
+		}
+
+		template <typename Iter, typename SynIter, typename Backend>
+		HYBRID static void receive( int_ src, Iter dst, SynIter && syn, snn_info info, Backend & bak )
+		{
+			if( accepts( dst ) ) deliver( src, dst, syn, info, bak );
 		}
 	};
 };

@@ -70,16 +70,28 @@ struct brunel_3pop : model
 			return false;
 		}
 
-		template <typename Iter, typename SynIter, typename Backend>
-		HYBRID static void receive( int_ src, Iter dst, SynIter &&, snn_info info, Backend & bak )
+		// receive() = accepts() ? deliver() : nothing (split so that the GPU may test the gate
+		// in the neuron update)
+		static constexpr bool gated_receive = true;
+
+		template <typename Iter>
+		HYBRID static bool accepts( Iter n )
 		{
-			using util::get;
-			if( get<Twait>( dst ) <= 0 )
-			{
-				float const Wex = 0.0001f * 50'000 / info.num_neurons;  // V
-				float const Win = -0.0005f * 50'000 / info.num_neurons; // V
-				bak.atomic_add( get<V>( dst ), src < num_exc_end( info ) ? Wex : Win );
-			}
+			return util::get<Twait>( n ) <= 0;
+		}
+
+		template <typename Iter, typename SynIter, typename Backend>
+		HYBRID static void deliver( int_ src, Iter dst, SynIter &&, snn_info info, Backend & bak )
+		{
+			float const Wex = 0.0001f * 50'000 / info.num_neurons;  // V
+			float const Win = -0.0005f * 50'000 / info.num_neurons; // V
+			bak.atomic_add( util::get<V>( dst ), src < num_exc_end( info ) ? Wex : Win );
+		}
+
+		template <typename Iter, typename SynIter, typename Backend>
+		HYBRID static void receive( int_ src, Iter dst, SynIter && syn, snn_info info, Backend & bak )
+		{
+			if( accepts( dst ) ) deliver( src, dst, syn, info, bak );
 		}
 	};
 };

@@ -21,16 +21,27 @@ struct brunel_3pop_stdp : model
 
 	struct neuron : brunel_3pop::neuron
 	{
+		// both halves of the split receive() are redefined (the base's deliver() has fixed weights)
 		template <typename Iter, typename SynIter, typename Backend>
-		HYBRID static void receive( int_, Iter dst, SynIter && syn, snn_info info, Backend & bak )
+		HYBRID static void deliver( int_, Iter dst, SynIter && syn, snn_info info, Backend & bak )
 		{
 			using util::get;
-			if( get<Twait>( dst ) <= 0 ) bak.atomic_add( get<V>( dst ), get<W>( syn ) * 50'000 / info.num_neurons );
+			bak.atomic_add( get<V>( dst ), get<W>( syn ) * 50'000 / info.num_neurons );
+		}
+
+		template <typename Iter, typename SynIter, typename Backend>
+		HYBRID static void receive( int_ src, Iter dst, SynIter && syn, snn_info info, Backend & bak )
+		{
+			if( accepts( dst ) ) deliver( src, dst, syn, info, bak );
 		}
 	};
 
 	struct synapse : ::spice::trace_synapse<float>
 	{
+		// plastic synapses join excitatory LIF neurons: after a spike at t, Twait = 20 closes the
+		// gate for the inputs of t + 1 .. t + 20 and the next spike comes at t + 20 at the earliest
+		static constexpr int refractory_steps = 19;
+
 		template <typename Iter, typename Backend>
 		HYBRID static void init( Iter syn, int_ src, int_, snn_info info, Backend & )
 		{
@@ -56,6 +67,19 @@ struct brunel_3pop_stdp : model
 			int_ const ne = brunel_3pop::num_exc_end( info );
 			return src >= np && src < ne && dst >= np && dst < ne;
 		}
+	};
+};
+
+// The same model without the refractory promise: the GPU cannot rely on a plastic synapse's
+// target ignoring the input of a delivery while that synapse's previous event is still being
+// applied, so each delivery applies the previous step's events of the rows it reads itself
+// (deferred events), and the update of a step runs after its delivery. Identical results; kept
+// so that both schedules stay exercised.
+struct brunel_3pop_stdp_deferred : brunel_3pop_stdp
+{
+	struct synapse : brunel_3pop_stdp::synapse
+	{
+		static constexpr int refractory_steps = 0;
 	};
 };
 } // namespace spice

@@ -32,6 +32,26 @@
 // naming the attributes receive() reads from its target (without it, none are assumed: reads
 // then go to device memory for every delivered event; with it, the delivering workgroup keeps
 // a copy of its tile's values of those attributes in LDS). Neither changes results.
+//
+// Optional split of receive() into a gate and an ungated input: a neuron type may declare
+//     static constexpr bool gated_receive = true;
+//     template <typename Iter> static bool accepts( Iter n );   reads only receive_reads
+//     static void deliver( src, dst_it, syn_it, info, backend ); reads nothing of the target
+// with receive() == (accepts(dst) ? deliver() : nothing). The GPU may then add a step's input
+// before the target's previous update has finished and leave the gate to the neuron update,
+// which sees the state receive() would have seen: with a delay of at least 2 steps the
+// delivery of step T + 1 runs beside the update of step T. A neuron declaring
+//     static constexpr std::uint32_t receive_reads = 0;
+// (receive() reads nothing of its target) allows the same without a gate.
+//
+// Optional trait of a neuron-trace synapse (trace_synapse below):
+//     static constexpr int refractory_steps = R;
+// promises that a neuron that is an endpoint of a plastic synapse and spikes at step t neither
+// spikes nor accepts input at steps t + 1 .. t + R. With R >= 2 a plastic synapse's events of
+// the two steps before an arrival of its source's spike concern a target that does not take
+// that spike's input (a post event), or cannot happen (a pre event), so the delivery may read
+// the weight while those events are still being applied; the GPU then applies every event in
+// the plasticity passes of its step and none in the delivery.
 #include <spice/snn_info.h>
 #include <spice/util/host_defines.h>
 #include <spice/util/meta.h>
@@ -164,6 +184,50 @@ struct receive_reads_of<N, std::void_t<decltype( N::receive_reads )>>
 // Bit I set <=> receive() reads neuron attribute I of its target.
 template <typename Neuron>
 inline constexpr std::uint32_t receive_reads_v = detail::receive_reads_of<Neuron>::value;
+
+namespace detail
+{
+template <typename N, typename = void>
+struct declares_reads : std::false_type
+{
+};
+template <typename N>
+struct declares_reads<N, std::void_t<decltype( N::receive_reads )>> : std::true_type
+{
+};
+template <typename N, typename = void>
+struct gated_receive_of : std::false_type
+{
+};
+template <typename N>
+struct gated_receive_of<N, std::void_t<decltype( N::gated_receive )>> : std::bool_constant<N::gated_receive>
+{
+};
+template <typename S, typename = void>
+struct refractory_steps_of
+{
+	static constexpr int value = 0;
+};
+template <typename S>
+struct refractory_steps_of<S, std::void_t<decltype( S::refractory_steps )>>
+{
+	static constexpr int value = S::refractory_steps;
+};
+} // namespace detail
+
+// The neuron splits receive() into accepts() and deliver().
+template <typename Neuron>
+inline constexpr bool gated_receive_v = detail::gated_receive_of<Neuron>::value;
+
+// Delivery may run before the target's previous update is done: receive() is split into a gate
+// and an ungated input, or is declared to read nothing of its target.
+template <typename Neuron>
+inline constexpr bool input_ahead_of_update_v =
+    gated_receive_v<Neuron> || ( detail::declares_reads<Neuron>::value && receive_reads_v<Neuron> == 0 );
+
+// Refractory steps promised for the endpoints of plastic neuron-trace synapses (0: none).
+template <typename Synapse>
+inline constexpr int refractory_steps_v = detail::refractory_steps_of<Synapse>::value;
 
 template <typename Model>
 inline constexpr bool is_plastic_v = Model::synapse::size > 0;

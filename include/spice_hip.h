@@ -28,7 +28,9 @@ extern "C" {
 
 /* Models compiled into the library. 0-3 are the reference's models
  * (spice/models/{vogels_abbott,brunel,brunel_with_plasticity,synth}.h); 4-5 are the
- * three-population Brunel variants of the benchmark configurations. */
+ * three-population Brunel variants of the benchmark configurations; 6 is model 5 without its
+ * promise of refractory plastic endpoints (the same dynamics, run on the two-kernel step whose
+ * deliveries apply the previous step's events of the rows they read). */
 enum spice_model_id
 {
 	SPICE_MODEL_VOGELS_ABBOTT = 0,
@@ -37,7 +39,8 @@ enum spice_model_id
 	SPICE_MODEL_SYNTH = 3,
 	SPICE_MODEL_BRUNEL_3POP = 4,
 	SPICE_MODEL_BRUNEL_3POP_STDP = 5,
-	SPICE_NUM_MODELS = 6
+	SPICE_MODEL_BRUNEL_3POP_STDP_DEFERRED = 6,
+	SPICE_NUM_MODELS = 7
 };
 
 enum spice_status
@@ -263,10 +266,13 @@ int spice_hip_run( spice_net * net, int64_t nsteps );
 
 /* Enqueues nsteps steps without graph capture, each kernel launched with start and stop HIP
  * events attached to its dispatch (hipExtLaunchKernel: the kernel's own duration, without the
- * launch gap) on the net's stream, waits, and adds the summed per-kernel device times in milliseconds to
- * kernel_ms[0] (spike delivery; for plastic models including the previous step's plasticity
- * work, which runs in the same kernel), kernel_ms[1] (neuron update) and kernel_ms[2]
- * (always 0: no separate plasticity kernel). Same results as spice_hip_run. Used for roofline
+ * launch gap) on the net's stream, waits, and adds the summed per-kernel device times in
+ * milliseconds to kernel_ms[0..2]. Two-kernel steps: kernel_ms[0] spike delivery (for plastic
+ * models including the previous step's plasticity work, which runs in the same kernel),
+ * kernel_ms[1] neuron update, kernel_ms[2] 0. Overlapped steps (models whose delivery may run
+ * before the target's update, delay >= 2): kernel_ms[0] the first step's delivery, kernel_ms[1]
+ * the last step's update, kernel_ms[2] the nsteps - 1 kernels that each run the update of a
+ * step beside the delivery of the next. Same results as spice_hip_run. Used for roofline
  * figures. */
 int spice_hip_run_profiled( spice_net * net, int64_t nsteps, double * kernel_ms );
 

@@ -29,7 +29,7 @@ class Edge(_c.Structure):
 
 
 MODEL_IDS = {"vogels_abbott": 0, "brunel": 1, "brunel_with_plasticity": 2, "synth": 3,
-             "brunel_3pop": 4, "brunel_3pop_stdp": 5}
+             "brunel_3pop": 4, "brunel_3pop_stdp": 5, "brunel_3pop_stdp_deferred": 6}
 NEURON_ATTRS = {
     "vogels_abbott": [("V", np.float32), ("Gex", np.float32), ("Gin", np.float32), ("Twait", np.int32)],
     "brunel": [("V", np.float32), ("Twait", np.int32)],
@@ -37,10 +37,12 @@ NEURON_ATTRS = {
     "synth": [("N", np.int32)],
     "brunel_3pop": [("V", np.float32), ("Twait", np.int32)],
     "brunel_3pop_stdp": [("V", np.float32), ("Twait", np.int32)],
+    "brunel_3pop_stdp_deferred": [("V", np.float32), ("Twait", np.int32)],
 }
 SYNAPSE_ATTRS = {
     "brunel_with_plasticity": [("W", np.float32), ("Zpre", np.float32), ("Zpost", np.float32)],
     "brunel_3pop_stdp": [("W", np.float32)],
+    "brunel_3pop_stdp_deferred": [("W", np.float32)],
 }
 
 _lib = None

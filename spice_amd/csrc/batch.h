@@ -118,7 +118,10 @@ public:
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
 		if( n == 0 ) return;
-		if( _graphs.empty() ) _graphs.capture( _s, [&]( int t ) { launch_step( t ); } );
+		if( _graphs.empty() )
+			_graphs.capture( _s, [&]( int t, int len ) {
+				for( int k = 0; k < len; k++ ) launch_step( t + k );
+			} );
 		_graphs.run( _s, _step, n, [&] { _step++; } );
 		for( auto * net : _nets ) net->advance_host( n );
 	}

@@ -7,7 +7,8 @@
 // HBM layout of one net:
 //   row_ptr  int64[N+1], col int32[E]           CSR adjacency, rows sorted by target
 //   na[a]    attribute a of every neuron          structure of arrays
-//   acc[a]   receive accumulator of attribute a   fp32 / int64 (fixed point) / int32
+//   acc[a]   receive accumulators of attribute a  [2][N] by step parity: fp32 / int64 (fixed
+//                        point) / int32
 //   sa[a]    attribute a of every synapse         CSR order (plastic models only)
 //   ring     int32[(D+1) * N], counts uint32[D+1] spike ids of the last D+1 steps
 //   hring    uint64[HWORDS * N]  spike-history ring: bit (t mod H) of word (t mod H)/64 of
@@ -25,7 +26,9 @@
 // eager plasticity (neuron-trace synapses), all [2][N] arrays by step parity:
 //   xtr/ytr  f32[2][N]   neuron-local pre/post traces after step T
 //   fl8      u8[2][N]    bit 0: spiked at T, bit 1: its spike arrives at T, bit 2: at T + 1
-//   fl_id    int32[2][N] rows whose spike arrived at T (and does not arrive at T + 1)
+//                        (bit 2 only for models whose deliveries apply deferred events)
+//   fl_id    int32[2][N] rows whose spike arrived at T (and, with deferred events, does not
+//                        arrive at T + 1)
 //   po_id    int32[2][N] neurons that spiked at T and have incoming plastic synapses
 //   tp_ptr   int64[N+1], tp_ent {src, offset in src's row}[P]: the P plastic synapses by target
 // both:
@@ -93,6 +96,8 @@ struct dview
 	std::int32_t fparts; // work items per flushed row
 	std::int32_t pparts; // work items per spiking target (eager plasticity)
 	std::int32_t rparts; // delivery workgroups per tile (the step's spike list split between them)
+	std::int32_t overlap; // 1: the update of step T runs beside the delivery of T + 1 (it writes
+	                      // the spike log's step offsets then; otherwise the delivery does)
 	std::uint32_t flags;
 	std::uint64_t seed;
 	float param[4]; // per-net model parameters (snn_info::param)

@@ -24,6 +24,7 @@ SPICE_MODEL_FACTORIES( brunel_with_plasticity )
 SPICE_MODEL_FACTORIES( synth )
 SPICE_MODEL_FACTORIES( brunel_3pop )
 SPICE_MODEL_FACTORIES( brunel_3pop_stdp )
+SPICE_MODEL_FACTORIES( brunel_3pop_stdp_deferred )
 #undef SPICE_MODEL_FACTORIES
 
 // Graph generation into caller-owned device memory behind spice_hip_generate_graph* (capi.hip).
@@ -83,6 +84,7 @@ inline net_iface * make_net( net_params const & p, csr_dev && g, hipStream_t s )
 		case SPICE_MODEL_SYNTH: return make_net_synth( p, std::move( g ), s );
 		case SPICE_MODEL_BRUNEL_3POP: return make_net_brunel_3pop( p, std::move( g ), s );
 		case SPICE_MODEL_BRUNEL_3POP_STDP: return make_net_brunel_3pop_stdp( p, std::move( g ), s );
+		case SPICE_MODEL_BRUNEL_3POP_STDP_DEFERRED: return make_net_brunel_3pop_stdp_deferred( p, std::move( g ), s );
 		default: throw std::invalid_argument( "unknown model id" );
 		}
 	}
@@ -106,6 +108,7 @@ inline batch_iface * make_batch( int model, std::vector<net_params> const & ps, 
 		case SPICE_MODEL_SYNTH: return make_batch_synth( ps, std::move( gs ), s );
 		case SPICE_MODEL_BRUNEL_3POP: return make_batch_brunel_3pop( ps, std::move( gs ), s );
 		case SPICE_MODEL_BRUNEL_3POP_STDP: return make_batch_brunel_3pop_stdp( ps, std::move( gs ), s );
+		case SPICE_MODEL_BRUNEL_3POP_STDP_DEFERRED: return make_batch_brunel_3pop_stdp_deferred( ps, std::move( gs ), s );
 		default: throw std::invalid_argument( "unknown model id" );
 		}
 	}
@@ -127,6 +130,7 @@ inline batch_iface * make_group( int model, std::vector<net_iface *> const & net
 	case SPICE_MODEL_SYNTH: return make_group_synth( nets, s );
 	case SPICE_MODEL_BRUNEL_3POP: return make_group_brunel_3pop( nets, s );
 	case SPICE_MODEL_BRUNEL_3POP_STDP: return make_group_brunel_3pop_stdp( nets, s );
+	case SPICE_MODEL_BRUNEL_3POP_STDP_DEFERRED: return make_group_brunel_3pop_stdp_deferred( nets, s );
 	default: throw std::invalid_argument( "unknown model id" );
 	}
 }

@@ -11,7 +11,7 @@
 //
 // Replay keeps at most INFLIGHT graph launches queued on the stream: before launch i the host
 // waits for launch i - INFLIGHT (an event recorded after each launch). A long run (100,000
-// steps are 3,125 launches of 64 kernels) thus never queues more than 64 x 64 dispatches
+// steps are 3,125 launches of up to 64 kernels) thus never queues more than 64 x 64 dispatches
 // ahead of the GPU, whatever the caller asks for, while a chunk of up to ~2,000 steps is
 // enqueued without the host waiting (so callers that drive several streams, like the server's
 // per-model batches, get all their chunks queued at once and running side by side).
@@ -53,17 +53,14 @@ public:
 		_launched = 0;
 	}
 
-	// launch(t) enqueues on `s` the kernels of a step of clock phase t & 3.
+	// launch(t, len) enqueues on `s` the kernels of len steps from a step of clock phase t & 3.
 	template <typename Launch>
 	void capture( hipStream_t s, Launch && launch )
 	{
 		try
 		{
-			for( int p = 0; p < CLOCK_SLOTS; p++ ) _one[p] = capture_one( s, [&] { launch( p ); } );
-			for( int k = 0; k < BLOCKS; k++ )
-				_blk[k] = capture_one( s, [&] {
-					for( int i = 0; i < ( 4 << k ); i++ ) launch( i );
-				} );
+			for( int p = 0; p < CLOCK_SLOTS; p++ ) _one[p] = capture_one( s, [&] { launch( p, 1 ); } );
+			for( int k = 0; k < BLOCKS; k++ ) _blk[k] = capture_one( s, [&] { launch( 0, 4 << k ); } );
 			for( auto & e : _ev ) SPICE_HIP( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
 		}
 		catch( ... )

@@ -358,8 +358,28 @@ __device__ inline void replay(
 // they touch no row that delivery reads, and they read only step T's traces and flags (arrays
 // by step parity), which the update of step T + 1 writes only after that kernel.
 
+//
+// A synapse type promising refractory endpoints (refractory_steps >= 2, see model.h) needs no
+// deferred events: the pre pass of step T takes every row whose spike arrives at T, the post
+// pass every other source, and the delivery only reads weights. A weight it reads may still
+// get an event of step T - 1 or T applied beside it, but only one whose target does not take
+// this delivery's input.
+
 // flag bits of fl8
 constexpr std::uint8_t F8_SPIKED = 1, F8_ARRIVED = 2, F8_ARRIVES_NEXT = 4;
+
+// Eager plasticity with every event applied by the passes of its step (no deferred events).
+template <typename Model>
+inline constexpr bool eager_nodefer_v =
+    is_plastic_v<Model> && has_neuron_traces_v<typename Model::synapse> && refractory_steps_v<typename Model::synapse> >= 2;
+
+// The update of step T may run beside the delivery of step T + 1 (given a delay of at least 2
+// steps): delivery reads nothing that the update writes -- no target state (receive() split
+// into a gate the update tests, or declared to read nothing) and, for plastic models, no weight
+// whose pending events matter.
+template <typename Model>
+inline constexpr bool overlap_capable_v =
+    input_ahead_of_update_v<typename Model::neuron> && ( !is_plastic_v<Model> || eager_nodefer_v<Model> );
 
 // ---- kernels -------------------------------------------------------------------------------
 
@@ -413,7 +433,6 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 #ifndef SPICE_UPDATE_BLOCK
 #define SPICE_UPDATE_BLOCK 256 // threads per neuron-update workgroup of a single net (64..256)
 #endif
-// update_block's per-wave compaction counters hold at most 4 waves
 static_assert( SPICE_UPDATE_BLOCK % 64 == 0 && SPICE_UPDATE_BLOCK >= 64 && SPICE_UPDATE_BLOCK <= 256,
                "SPICE_UPDATE_BLOCK must be 64, 128, 192 or 256 threads" );
 #ifndef SPICE_BATCH_ITEMS_PER_CU
@@ -615,6 +634,18 @@ __device__ inline void segments_of( std::int32_t const * pre, std::int32_t cnt, 
 	}
 }
 
+// One delivered event: receive(), or for a neuron with a split receive() its ungated half
+// deliver() (the neuron update then tests the gate, accepts(), on the state receive() would
+// have read; a target that does not accept drops the whole step's input).
+template <typename N, typename... A>
+__device__ inline void deliver_event( A &&... a )
+{
+	if constexpr( gated_receive_v<N> )
+		N::deliver( static_cast<A &&>( a )... );
+	else
+		N::receive( static_cast<A &&>( a )... );
+}
+
 // Delivers the spikes of step T - D, one workgroup per tile of tile_w target neurons:
 // the workgroup stages a chunk of spikes (id, segment of the row inside the tile, and for
 // plastic models the row's lazy state) in LDS, forms the segments' prefix sums, and walks
@@ -622,8 +653,9 @@ __device__ inline void segments_of( std::int32_t const * pre, std::int32_t cnt, 
 // entry's segment found from a bitmap of segment starts, or for very large chunks by binary
 // search). receive() accumulates into
 // LDS with LDS atomics; at the end the tile's sums go to the global accumulators with plain
-// stores: one workgroup owns each tile, so no global atomics are needed.
-template <typename Model, bool FIXED, int BS>
+// stores: one workgroup owns each tile, so no global atomics are needed. AHEAD: the kernel's
+// clock is step T - 1's (the overlapped step kernel delivers the next step beside the update).
+template <typename Model, bool FIXED, int BS, bool AHEAD = false>
 __device__ inline void receive_tile( dview const & v, std::int32_t work )
 {
 	// work item = (tile, part): the tile's spike list is split into v.rparts parts, each
@@ -636,7 +668,8 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	constexpr std::uint32_t rmask = receive_mask_v<N>;
 	constexpr int nacc = __builtin_popcount( rmask ) > 0 ? __builtin_popcount( rmask ) : 1;
 
-	constexpr std::uint32_t smask = staged_mask<N>();
+	// a split receive() reads nothing of its target: no attribute is staged
+	constexpr std::uint32_t smask = gated_receive_v<N> ? 0u : staged_mask<N>();
 	constexpr int nstaged = __builtin_popcount( smask );
 	using word = acc_word<N, FIXED>;
 	__shared__ word acc[nacc * TILE_MAX];
@@ -657,20 +690,25 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	__shared__ std::uint64_t seg_map[SW]; // low half: start bitmap word, high half: its first event's segment
 	__shared__ std::uint64_t seg_total;
 	constexpr bool eager = plastic && has_neuron_traces_v<S>;
-	__shared__ std::int32_t seg_synced[plastic && !eager ? BS : 1];
-	__shared__ std::int32_t seg_pend[plastic ? BS : 1]; // eager: the row's spike also arrived at T - 1
-	__shared__ float seg_xf[eager ? BS : 1];            // eager: the row's pre trace after T - 1
-	__shared__ std::uint8_t tile_f8[eager ? TILE_MAX : 1]; // eager: the tile's step T - 1 flags
-	__shared__ float tile_y[eager ? TILE_MAX : 1];         // eager: the tile's post traces after T - 1
+	constexpr bool lazy = plastic && !eager;
+	constexpr bool deferred = eager && !eager_nodefer_v<Model>; // delivery applies step T - 1's events
+	// record and target id of each entry loaded without branches (entries past the chunk's end
+	// read its last entry and are ignored)
+	constexpr bool flat = !plastic || ( eager && !deferred );
+	__shared__ std::int32_t seg_synced[lazy ? BS : 1];
+	__shared__ std::int32_t seg_pend[lazy || deferred ? BS : 1]; // deferred: the row's spike also arrived at T - 1
+	__shared__ float seg_xf[deferred ? BS : 1];                  // deferred: the row's pre trace after T - 1
+	__shared__ std::uint8_t tile_f8[deferred ? TILE_MAX : 1];    // deferred: the tile's step T - 1 flags
+	__shared__ float tile_y[deferred ? TILE_MAX : 1];            // deferred: the tile's post traces after T - 1
 	__shared__ std::uint64_t wsum[BS / 64];
 
-	std::int64_t const T = v.ctl->step;
+	std::int64_t const T = v.ctl->step + ( AHEAD ? 1 : 0 );
 	int const D1 = v.D + 1;
 	int const slot = int( ( T + 1 ) % D1 ); // ring entry of step T - D
 	std::int32_t const nall = T < v.D ? 0 : std::int32_t( v.counts[slot] );
 	std::int32_t const s_beg = std::int32_t( std::int64_t( nall ) * part / v.rparts );
 	std::int32_t const nspk = std::int32_t( std::int64_t( nall ) * ( part + 1 ) / v.rparts ) - s_beg;
-	if( v.slog && work == 0 && threadIdx.x == 0 )
+	if( !v.overlap && v.slog && work == 0 && threadIdx.x == 0 )
 	{
 		// spike log: step T's ids start where step T-1's (final by now) end
 		std::int64_t const k = T - v.slog->t0;
@@ -682,15 +720,15 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	snn_info const info = info_of( v );
 	int const tid = threadIdx.x;
 	unsigned long long ev = 0, pev = 0;
-	// eager: step T - 1's events of the delivered rows were left to this delivery, unless a
+	// deferred: step T - 1's events of the delivered rows were left to this delivery, unless a
 	// synchronization (state export) has applied them already
-	bool const catch_up = eager && T >= 1 && *v.caught != T - 1;
+	bool const catch_up = deferred && T >= 1 && *v.caught != T - 1;
 	std::int64_t const prev = ( ( T - 1 ) & 1 ) * std::int64_t( v.N ); // step T - 1's arrays
 	{
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
 		for( int x = tid; x < nacc * TILE_MAX; x += BS ) acc[x] = word( 0 );
-		if constexpr( eager )
+		if constexpr( deferred )
 			if( catch_up ) // what step T - 1's deferred events read of their targets
 				for( int x = tid; x < tw; x += BS )
 				{
@@ -698,7 +736,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					tile_y[x] = v.ytr[prev + t0 + x];
 				}
 		// the tile's values of the attributes receive() reads (they do not change during
-		// delivery: the neuron update runs after it)
+		// delivery: the neuron update runs after it; a split receive() reads none)
 		util::for_n<N::size>( [&]( auto I ) {
 			if constexpr( ( ( smask >> I ) & 1u ) != 0 )
 			{
@@ -723,12 +761,12 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 				std::int32_t const a = pv[0], b = pv[1];
 				beg = v.row_ptr[src] + a;
 				len = b - a;
-				if constexpr( eager )
+				if constexpr( deferred )
 				{
 					pend = catch_up && ( v.fl8[prev + src] & F8_ARRIVED );
 					xf = catch_up ? v.xtr[prev + src] : 0.0f;
 				}
-				else if constexpr( plastic )
+				else if constexpr( lazy )
 				{
 					synced = v.synced[src];
 					pend = v.pend[src];
@@ -745,12 +783,12 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 			{
 				seg[c] = seg_rec{ beg - pre, src, 0 };
 				seg_pre[c] = pre;
-				if constexpr( eager )
+				if constexpr( deferred )
 				{
 					seg_pend[c] = pend;
 					seg_xf[c] = xf;
 				}
-				else if constexpr( plastic )
+				else if constexpr( lazy )
 				{
 					seg_synced[c] = synced;
 					seg_pend[c] = pend;
@@ -793,12 +831,12 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 				}
 				else
 					segments_of<U>( seg_pre, nseg, ff, kk );
-				if constexpr( !plastic )
+				if constexpr( flat )
 				{
 					// all U records, then all U target ids, without branches: an entry past the
 					// chunk's end reads the chunk's last entry (a valid address) and is ignored
-					// (plastic models keep the branches: the extra live registers cost them a wave
-					// per SIMD)
+					// (models whose delivery applies plasticity keep the branches: the extra live
+					// registers cost them a wave per SIMD)
 					seg_rec r[U];
 #pragma unroll
 					for( int u = 0; u < U; u++ ) r[u] = seg[kk[u]];
@@ -826,7 +864,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 						j[u] = ok[u] ? lds<( SPICE_NT & 4 ) != 0>( v.col + e[u] ) : 0;
 					}
 				}
-				if constexpr( eager )
+				if constexpr( deferred )
 				{
 					// neuron-trace synapses: step T - 1's event of each plastic synapse (the row's
 					// spike arrived at T - 1, or the target spiked at T - 1), then delivery. The
@@ -864,7 +902,25 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 							store_changed<S, ( SPICE_NT & 4 ) != 0>( before, sy[u], v.sa, e[u] );
 							pev++;
 						}
-						N::receive( s, dst, itr, info, bak );
+						deliver_event<N>( s, dst, itr, info, bak );
+					}
+				}
+				else if constexpr( eager )
+				{
+					// neuron-trace synapses without deferred events: the synapse state of all U
+					// entries (valid addresses, see above), then delivery
+					typename S::tuple_t sy[U];
+#pragma unroll
+					for( int u = 0; u < U; u++ ) load_tuple<S, ( SPICE_NT & 4 ) != 0>( sy[u], v.sa, e[u] );
+#pragma unroll
+					for( int u = 0; u < U; u++ )
+					{
+						if( !ok[u] ) continue;
+						std::int32_t const s = sv[u];
+						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
+						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc, stage };
+						reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
+						deliver_event<N>( s, dst, itr, info, bak );
 					}
 				}
 				else if constexpr( !plastic )
@@ -887,7 +943,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 						iter_v dst{ &v, j[u], t0, acc, {} };
 #pragma unroll
 						for( int b = 0; b < iter_v::NS; b++ ) dst.sval[b] = svals[u][b];
-						N::receive( s, dst, no_syn_iter{}, info, bak );
+						deliver_event<N>( s, dst, no_syn_iter{}, info, bak );
 					}
 				}
 				else
@@ -911,10 +967,10 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 							store_tuple<S>( sy, v.sa, e[u] );
 							pev++;
 						}
-						N::receive( s, dst, itr, info, bak );
+						deliver_event<N>( s, dst, itr, info, bak );
 					}
 					else
-						N::receive( s, dst, no_syn_iter{}, info, bak );
+						deliver_event<N>( s, dst, no_syn_iter{}, info, bak );
 				}
 				}
 			};
@@ -925,10 +981,11 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 			__syncthreads(); // segments are overwritten by the next chunk
 		}
 
-		// the tile's sums -> global accumulators (zero here). One part: this workgroup owns
-		// them, plain stores; several parts: atomic adds (integer and thus exact in fixed-point
-		// mode; in fp32 mode the parts' order varies, as LDS atomic order already does)
+		// the tile's sums -> step T's global accumulators (zero here). One part: this workgroup
+		// owns them, plain stores; several parts: atomic adds (integer and thus exact in
+		// fixed-point mode; in fp32 mode the parts' order varies, as LDS atomic order already does)
 		bool const owned = v.rparts == 1;
+		std::int64_t const apar = ( T & 1 ) * std::int64_t( v.N );
 		util::for_n<N::size>( [&]( auto I ) {
 			if constexpr( ( ( rmask >> I ) & 1u ) != 0 )
 			{
@@ -940,11 +997,11 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					A const val = src[x];
 					if( val == A( 0 ) ) continue;
 					if( owned )
-						st<A>( v.acc[I], t0 + x, val );
+						st<A>( v.acc[I], apar + t0 + x, val );
 					else if constexpr( std::is_same_v<A, long long> )
-						atomicAdd( reinterpret_cast<unsigned long long *>( v.acc[I] ) + t0 + x, static_cast<unsigned long long>( val ) );
+						atomicAdd( reinterpret_cast<unsigned long long *>( v.acc[I] ) + apar + t0 + x, static_cast<unsigned long long>( val ) );
 					else
-						atomicAdd( static_cast<A *>( v.acc[I] ) + t0 + x, val );
+						atomicAdd( static_cast<A *>( v.acc[I] ) + apar + t0 + x, val );
 				}
 			}
 		} );
@@ -990,6 +1047,34 @@ __global__ __launch_bounds__( BS ) void k_receive_tiled( dview v, std::int32_t g
 		plasticity_prev_block<Model, FIXED>( v, blockIdx.x - grecv, gridDim.x - grecv );
 }
 
+template <typename Model, bool FIXED>
+__device__ inline void update_block( dview const & v, std::int64_t blk, std::int64_t slot );
+
+// The overlapped step (overlap_capable_v models, delay >= 2), clock of step T: workgroups
+// [0, nrecv) deliver step T + 1, [nrecv, nrecv + nplast) run step T - 1's plasticity passes,
+// the rest update the neurons of step T (blockDim.x neurons each; statistics slots from
+// upd_slot0). None reads what another writes: the accumulators, plasticity lists, traces and
+// flags are kept by step parity, the delivery reads ring entries of earlier steps and no target
+// state (the update tests receive()'s gate), and the passes touch only weights whose targets
+// do not take that delivery's input. With nrecv = 0 the kernel ends a run of overlapped steps
+// (the last update, beside the passes of the step before it).
+template <typename Model, bool FIXED, int BS>
+__global__ __launch_bounds__( BS ) void k_step_overlap( dview v, std::int32_t nrecv, std::int32_t nplast, std::int32_t upd_slot0 )
+{
+	std::int32_t const b = blockIdx.x;
+	if( b < nrecv )
+	{
+		std::int32_t const nwork = v.ntiles * v.rparts;
+		for( std::int32_t w = b; w < nwork; w += nrecv ) receive_tile<Model, FIXED, BS, true>( v, w );
+	}
+	else if( b < nrecv + nplast )
+	{
+		if constexpr( is_plastic_v<Model> ) plasticity_prev_block<Model, FIXED>( v, b - nrecv, nplast );
+	}
+	else
+		update_block<Model, FIXED>( v, b - nrecv - nplast, upd_slot0 + ( b - nrecv - nplast ) );
+}
+
 // Batched delivery: global work item gt belongs to net tile_net[gt] (local work item
 // tile_local[gt] = tile * rparts + part); the net's view is read from device memory. Plastic
 // models: workgroup grecv + b runs plasticity block block_local[b] of the block_count[b]
@@ -1013,8 +1098,10 @@ __global__ __launch_bounds__( BS ) void k_batch_receive( dview const * views, st
 
 
 
+// Neuron update of block blk (blockDim.x <= 1024 neurons) at step T; statistics go to slot
+// `slot`.
 template <typename Model, bool FIXED>
-__device__ inline void update_block( dview const & v, std::int64_t blk )
+__device__ inline void update_block( dview const & v, std::int64_t blk, std::int64_t slot )
 {
 	using N = typename Model::neuron;
 	using S = typename Model::synapse;
@@ -1060,7 +1147,7 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 		if constexpr( plastic )
 		{
 			if( T >= v.D ) h_arr = v.hring[( ( ( T - v.D ) & ( HISTORY - 1 ) ) >> 6 ) * N64 + i];
-			if( T + 1 >= v.D && v.D != 1 ) h_next = v.hring[( ( ( T + 1 - v.D ) & ( HISTORY - 1 ) ) >> 6 ) * N64 + i];
+			if( !eager_nodefer_v<Model> && T + 1 >= v.D && v.D != 1 ) h_next = v.hring[( ( ( T + 1 - v.D ) & ( HISTORY - 1 ) ) >> 6 ) * N64 + i];
 			h_now = v.hring[hcur];
 			has_plastic = v.hasp[i];
 			if constexpr( eager )
@@ -1077,21 +1164,27 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 				pend0 = v.pend[i];
 			}
 		}
+		// step T's inputs (accumulators of step parity T & 1); a split receive()'s gate is tested
+		// here, on the state the delivery's receive() would have read: a target that does not
+		// accept drops the step's input
+		bool open = true;
+		if constexpr( gated_receive_v<N> ) open = N::accepts( reg_iter<typename N::tuple_t>{ s, std::uint32_t( i ) } );
+		std::int64_t const ai = ( T & 1 ) * N64 + i;
 		util::for_n<N::size>( [&]( auto I ) {
 			if constexpr( ( ( rmask >> I ) & 1u ) != 0 )
 			{
 				using A = typename N::template type<I>;
 				if constexpr( std::is_floating_point_v<A> && FIXED )
 				{
-					long long const a = ld<long long>( v.acc[I], i );
-					st<long long>( v.acc[I], i, 0 );
-					std::get<I>( s ) = std::get<I>( s ) + float( a ) * __builtin_ldexpf( 1.0f, -v.shift );
+					long long const a = ld<long long>( v.acc[I], ai );
+					st<long long>( v.acc[I], ai, 0 );
+					if( open ) std::get<I>( s ) = std::get<I>( s ) + float( a ) * __builtin_ldexpf( 1.0f, -v.shift );
 				}
 				else
 				{
-					A const a = ld<A>( v.acc[I], i );
-					st<A>( v.acc[I], i, A( 0 ) );
-					std::get<I>( s ) = A( std::get<I>( s ) + a );
+					A const a = ld<A>( v.acc[I], ai );
+					st<A>( v.acc[I], ai, A( 0 ) );
+					if( open ) std::get<I>( s ) = A( std::get<I>( s ) + a );
 				}
 			}
 		} );
@@ -1105,8 +1198,10 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 		{
 			// this neuron's spike of step T - D reaches its synapses now
 			bool const arrived = T >= v.D && ( ( h_arr >> ( ( T - v.D ) & 63 ) ) & 1ull );
-			// and its spike of step T + 1 - D at the next step (recorded already unless D = 1)
-			bool const arrives_next = T + 1 >= v.D && ( v.D == 1 ? spiked : ( ( h_next >> ( ( T + 1 - v.D ) & 63 ) ) & 1ull ) != 0 );
+			// and its spike of step T + 1 - D at the next step (recorded already unless D = 1); only
+			// models with deferred events (or lazy plasticity) leave such rows to that delivery
+			bool const arrives_next = !eager_nodefer_v<Model> && T + 1 >= v.D &&
+			                          ( v.D == 1 ? spiked : ( ( h_next >> ( ( T + 1 - v.D ) & 63 ) ) & 1ull ) != 0 );
 			{ // record this step in the history ring (overwriting step T - HISTORY)
 				unsigned long long const bit = 1ull << ( T & 63 );
 				unsigned long long const nw = spiked ? ( h_now | bit ) : ( h_now & ~bit );
@@ -1121,9 +1216,9 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 				v.xtr[cur + i] = fmaf( x_prev, d, arrived ? 1.0f : 0.0f );
 				v.ytr[cur + i] = fmaf( y_prev, d, spiked ? 1.0f : 0.0f );
 				v.fl8[cur + i] = std::uint8_t( ( spiked ? F8_SPIKED : 0 ) | ( arrived ? F8_ARRIVED : 0 ) | ( arrives_next ? F8_ARRIVES_NEXT : 0 ) );
-				// pre pass: a row whose spike arrives now, unless it arrives again at T + 1 (then
-				// that delivery applies this step's events of the row); post pass: a spiking
-				// neuron with incoming plastic synapses
+				// pre pass: a row whose spike arrives now, unless (deferred events) it arrives again
+				// at T + 1 and that delivery applies this step's events of the row; post pass: a
+				// spiking neuron with incoming plastic synapses
 				flush = arrived && !arrives_next && has_plastic;
 				post = spiked && in_hi > in_lo;
 			}
@@ -1167,8 +1262,9 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 
 	// Spike compaction: ballot + popcount per wave, prefix over the block's waves in LDS, one
 	// atomic per block on the step's counter, mbcnt gives each spiking lane its slot.
-	__shared__ unsigned wcount[4][3];
+	__shared__ unsigned wcount[16][3]; // at most 1024 threads
 	__shared__ unsigned bbase[3];
+	__shared__ std::uint64_t lbase; // overlapped step: where step T's ids start in the spike log
 	int const w = threadIdx.x >> 6;
 	std::uint64_t const m = ballot( spiked );
 	std::uint64_t const fm = plastic ? ballot( flush ) : 0ull;
@@ -1196,7 +1292,27 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 		bbase[0] = t0 ? atomicAdd( &v.counts[wslot], t0 ) : 0u;
 		if constexpr( plastic ) bbase[1] = t1 ? atomicAdd( &v.fl_count[T & 3], t1 ) : 0u;
 		if constexpr( eager ) bbase[2] = t2 ? atomicAdd( &v.po_count[T & 3], t2 ) : 0u;
-		if( t0 ) v.bstats[blk * NSTAT + ST_SPIKES] += t0;
+		if( t0 ) v.bstats[slot * NSTAT + ST_SPIKES] += t0;
+		if( v.slog )
+		{
+			// where step T's ids start in the log: written by this step's delivery, or (the
+			// overlapped step, whose delivery of step T ran beside step T - 1's update) here,
+			// from step T - 1's offset and count, both final (delay >= 2: this update does not
+			// clear step T - 1's count)
+			std::int64_t const k = T - v.slog->t0;
+			std::uint64_t base = 0;
+			if( k >= 0 && k < v.slog->nsteps )
+			{
+				if( !v.overlap )
+					base = v.slog_off[k];
+				else
+				{
+					base = k == 0 ? 0ull : v.slog_off[k - 1] + v.counts[( T - 1 ) % D1];
+					if( blk == 0 ) v.slog_off[k] = base;
+				}
+			}
+			lbase = base;
+		}
 	}
 	__syncthreads();
 	if( spiked )
@@ -1205,11 +1321,10 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 		v.ring[std::int64_t( wslot ) * v.N + o] = std::int32_t( i );
 		if( v.slog )
 		{
-			// the step's offset was written by this step's receive kernel
 			std::int64_t const k = T - v.slog->t0;
 			if( k >= 0 && k < v.slog->nsteps )
 			{
-				std::uint64_t const p = v.slog_off[k] + o;
+				std::uint64_t const p = lbase + o;
 				if( p < v.slog->cap ) v.slog_ids[p] = std::int32_t( i );
 			}
 		}
@@ -1236,14 +1351,14 @@ __device__ inline void update_block( dview const & v, std::int64_t blk )
 template <typename Model, bool FIXED>
 __global__ __launch_bounds__( SPICE_UPDATE_BLOCK ) void k_update( dview v )
 {
-	update_block<Model, FIXED>( v, blockIdx.x );
+	update_block<Model, FIXED>( v, blockIdx.x, blockIdx.x );
 }
 
 // Batched neuron update: workgroup b updates block block_local[b] of net block_net[b].
 template <typename Model, bool FIXED>
 __global__ __launch_bounds__( 256 ) void k_batch_update( dview const * views, std::int32_t const * block_net, std::int32_t const * block_local )
 {
-	update_block<Model, FIXED>( views[block_net[blockIdx.x]], block_local[blockIdx.x] );
+	update_block<Model, FIXED>( views[block_net[blockIdx.x]], block_local[blockIdx.x], block_local[blockIdx.x] );
 }
 
 // Eager plasticity of step T (neuron-trace synapses): the pre pass and the post pass as one

@@ -107,6 +107,7 @@ class hip_net final : public net_iface
 	using S = typename Model::synapse;
 	static constexpr bool plastic = is_plastic_v<Model>;
 	static constexpr bool traced = plastic && has_neuron_traces_v<S>;
+	static constexpr bool nodefer = eager_nodefer_v<Model>; // every eager event applied by the passes
 
 public:
 	hip_net( net_params const & p, csr_dev && g, hipStream_t stream )
@@ -121,6 +122,8 @@ public:
 			throw std::invalid_argument( "plastic models support delays of at most 255 steps" );
 		if( _g.n < 1 || _g.n > INT_MAX - 1024 ) throw std::invalid_argument( "invalid neuron count" );
 		static_assert( N::size <= MAXA && S::size <= MAXA, "at most 16 neuron/synapse attributes" );
+		// the update of step T runs beside the delivery of step T + 1 (see k_step_overlap)
+		_overlap = overlap_capable_v<Model> && p.delay >= 2;
 		allocate();
 		initialize();
 	}
@@ -136,7 +139,7 @@ public:
 	void step( std::int32_t ** dspk, std::uint32_t ** dcnt ) override
 	{
 		std::int64_t const t = _step;
-		launch_step( t );
+		launch_steps( t, 1 );
 		tick();
 		int const slot = int( t % ( _p.delay + 1 ) );
 		if( dspk ) *dspk = _ring.data() + std::int64_t( slot ) * _g.n;
@@ -148,16 +151,26 @@ public:
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
 		if( n == 0 ) return;
-		if( _graphs.empty() ) _graphs.capture( _s, [&]( int t ) { launch_step( t ); } );
+		if( _graphs.empty() ) _graphs.capture( _s, [&]( int t, int len ) { launch_steps( t, len ); } );
 		_graphs.run( _s, _step, n, [&] { tick(); } );
 	}
 
 	// Runs n steps eagerly, each kernel with HIP events attached to its dispatch on the net's
-	// stream, and adds the summed kernel durations (ms) to kernel_ms[0..2] = receive, update,
-	// plasticity (0: plasticity runs inside the receive kernel).
+	// stream, and adds the summed kernel durations (ms) to kernel_ms[0..2]: receive (delivery,
+	// with the previous step's plasticity), update, and 0; overlapped steps: the first delivery,
+	// the last update (with the passes of the step before it), and the n - 1 overlapped step
+	// kernels.
 	void run_profiled( std::int64_t n, double * kernel_ms ) override
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
+		if( _overlap )
+		{
+			if( fixed() )
+				run_profiled_overlap<true>( n, kernel_ms );
+			else
+				run_profiled_overlap<false>( n, kernel_ms );
+			return;
+		}
 		constexpr int batch = 128, ev_per_step = 4;
 		synchronize(); // profiled steps run serially on the main stream
 		std::vector<hipEvent_t> ev( batch * ev_per_step, nullptr );
@@ -535,6 +548,7 @@ private:
 	int _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
 	int _grid_upd = 0; // workgroups of this net's own update kernel (SPICE_UPDATE_BLOCK threads)
 	int _grid_plast = 0; // plasticity workgroups of the delivery kernel
+	bool _overlap = false; // steps run as k_step_overlap kernels
 	launch_cfg _lc{};
 	dbuf<std::int32_t> _pivots;
 	std::int64_t _step = 0;
@@ -577,7 +591,7 @@ private:
 			if constexpr( ( ( receive_mask_v<N> >> I ) & 1u ) != 0 )
 			{
 				std::size_t const asz = ( std::is_floating_point_v<A> && true ) ? ( fixed() ? 8 : sizeof( A ) ) : sizeof( A );
-				_acc[I].resize( n * asz );
+				_acc[I].resize( 2 * n * asz ); // by step parity
 				SPICE_HIP( hipMemsetAsync( _acc[I].data(), 0, _acc[I].bytes(), _s ) );
 			}
 		} );
@@ -632,6 +646,7 @@ private:
 		v.D = _p.delay;
 		v.width = std::int32_t( std::min<std::int64_t>( _g.width, INT_MAX ) );
 		v.shift = 40;
+		v.overlap = _overlap ? 1 : 0;
 		v.DT = _p.dt;
 		v.flags = _p.flags;
 		v.seed = _p.seed;
@@ -796,10 +811,12 @@ private:
 		_grid_plast = plastic ? std::max( 1, _grid_flush * 256 / c.recv_block ) : 0; // the same waves as flush_grid
 		_lc = c;
 		// statistics slots: one per workgroup of the widest step kernel (receive: per work item;
-		// eager plasticity: after those, one per plasticity workgroup or synchronization block)
-		// (and enough for the update kernel's blocks)
+		// eager plasticity: after those, one per plasticity workgroup or synchronization block;
+		// the overlapped step's update blocks after those), and enough for the update kernel's
+		// blocks
 		std::size_t const slots =
-		    std::size_t( std::max( { _grid_update, _grid_upd, 2 * _grid_flush, 2 * _grid_tiles, v.ntiles * v.rparts + std::max( _grid_plast, _grid_flush ) } ) ) * NSTAT;
+		    std::size_t( std::max( { _grid_update, _grid_upd, 2 * _grid_flush, 2 * _grid_tiles,
+		                             upd_slot0() + int( ( n + c.recv_block - 1 ) / c.recv_block ) } ) ) * NSTAT;
 		if( slots > _bstats.size() )
 		{
 			std::vector<unsigned long long> keep( NSTAT, 0ull );
@@ -836,6 +853,105 @@ private:
 			SPICE_LAUNCH( k_init_synapses<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
 		}
 		SPICE_HIP( hipStreamSynchronize( _s ) );
+	}
+
+	// First statistics slot of the overlapped step's update blocks (after the delivery work
+	// items and the plasticity workgroups).
+	int upd_slot0() const { return _v.ntiles * _v.rparts + std::max( _grid_plast, _grid_flush ); }
+
+	// Enqueues steps t .. t + len - 1. Two kernels per step, or (overlapped steps) the delivery
+	// of step t, then one k_step_overlap per step boundary (update of T, delivery of T + 1,
+	// passes of T - 1), then the last update with the passes of the step before it: len + 1
+	// kernels, and as with the two-kernel step the passes of the last step are left to the
+	// next delivery kernel (or a synchronization).
+	void launch_steps( std::int64_t t, std::int64_t len )
+	{
+		if( !_overlap )
+		{
+			for( std::int64_t k = 0; k < len; k++ ) launch_step( t + k );
+			return;
+		}
+		for( std::int64_t k = 0; k <= len; k++ )
+		{
+			if( fixed() )
+				launch_overlap_kernel<true>( t, len, k, nullptr, nullptr );
+			else
+				launch_overlap_kernel<false>( t, len, k, nullptr, nullptr );
+		}
+	}
+
+	// Kernel k (0 .. len) of the overlapped sequence of steps t .. t + len - 1 (see
+	// launch_steps), with HIP events attached to its dispatch when start is not null.
+	template <bool F>
+	void launch_overlap_kernel( std::int64_t t, std::int64_t len, std::int64_t k, hipEvent_t start, hipEvent_t stop )
+	{
+		int const bs = _lc.recv_block;
+		if( k == 0 )
+		{
+			// delivery of step t only: step t - 1's passes run beside the first update
+			dview const & v = _vs[t & ( CLOCK_SLOTS - 1 )];
+			std::int32_t const tiles = _grid_tiles;
+			auto go = [&]( auto b ) {
+				constexpr int BS = decltype( b )::value;
+				if( start )
+					launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, BS> ), unsigned( tiles ), BS, start, stop, v, tiles );
+				else
+					SPICE_LAUNCH( ( k_receive_tiled<Model, F, BS><<<unsigned( tiles ), BS, 0, _s>>>( v, tiles ) ) );
+			};
+			bs == 256 ? go( std::integral_constant<int, 256>{} ) : bs == 512 ? go( std::integral_constant<int, 512>{} ) : go( std::integral_constant<int, 1024>{} );
+			return;
+		}
+		std::int64_t const T = t + k - 1; // this kernel updates step T
+		dview const & v = _vs[T & ( CLOCK_SLOTS - 1 )];
+		std::int32_t const nrecv = k < len ? _grid_tiles : 0; // the last kernel delivers nothing
+		std::int32_t const nplast = _grid_plast, s0 = upd_slot0();
+		unsigned const grid = unsigned( nrecv + nplast + int( ( _g.n + bs - 1 ) / bs ) );
+		auto go = [&]( auto b ) {
+			constexpr int BS = decltype( b )::value;
+			if( start )
+				launch_timed( reinterpret_cast<void const *>( &k_step_overlap<Model, F, BS> ), grid, BS, start, stop, v, nrecv, nplast, s0 );
+			else
+				SPICE_LAUNCH( ( k_step_overlap<Model, F, BS><<<grid, BS, 0, _s>>>( v, nrecv, nplast, s0 ) ) );
+		};
+		bs == 256 ? go( std::integral_constant<int, 256>{} ) : bs == 512 ? go( std::integral_constant<int, 512>{} ) : go( std::integral_constant<int, 1024>{} );
+	}
+
+	// run_profiled for overlapped steps: kernel 0 -> kernel_ms[0], kernels 1 .. n - 1 ->
+	// kernel_ms[2], kernel n -> kernel_ms[1].
+	template <bool F>
+	void run_profiled_overlap( std::int64_t n, double * kernel_ms )
+	{
+		if( n == 0 ) return;
+		constexpr int batch = 128;
+		synchronize();
+		std::int64_t const t0 = _step;
+		std::vector<hipEvent_t> ev( 2 * batch, nullptr );
+		try
+		{
+			for( auto & e : ev ) SPICE_HIP( hipEventCreate( &e ) );
+			for( std::int64_t done = 0; done <= n; )
+			{
+				int const k = int( std::min<std::int64_t>( batch, n + 1 - done ) );
+				for( int i = 0; i < k; i++ ) launch_overlap_kernel<F>( t0, n, done + i, ev[2 * i], ev[2 * i + 1] );
+				SPICE_HIP( hipStreamSynchronize( _s ) );
+				for( int i = 0; i < k; i++ )
+				{
+					float ms = 0;
+					SPICE_HIP( hipEventElapsedTime( &ms, ev[2 * i], ev[2 * i + 1] ) );
+					std::int64_t const idx = done + i;
+					kernel_ms[idx == 0 ? 0 : idx == n ? 1 : 2] += ms;
+				}
+				done += k;
+			}
+		}
+		catch( ... )
+		{
+			for( auto & e : ev )
+				if( e ) (void)hipEventDestroy( e );
+			throw;
+		}
+		for( auto & e : ev ) (void)hipEventDestroy( e );
+		for( std::int64_t i = 0; i < n; i++ ) tick();
 	}
 
 	// Enqueues the kernels of step `t` (t mod 4 selects the clock slot): delivery (plastic
@@ -913,7 +1029,8 @@ private:
 						SPICE_LAUNCH( k_stdp<Model, true><<<_grid_flush, 256, 0, _s>>>( v, now, slot0 ) );
 					else
 						SPICE_LAUNCH( k_stdp<Model, false><<<_grid_flush, 256, 0, _s>>>( v, now, slot0 ) );
-					SPICE_LAUNCH( k_sync_eager<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, now ) );
+					// deferred events of the rows arriving next (none when the passes apply all)
+					if constexpr( !nodefer ) SPICE_LAUNCH( k_sync_eager<Model><<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( _v, now ) );
 				}
 				else
 				{

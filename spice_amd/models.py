@@ -10,6 +10,8 @@ MODELS = {
     "synth": (3, [("N", np.int32)], []),
     "brunel_3pop": (4, [("V", np.float32), ("Twait", np.int32)], []),
     "brunel_3pop_stdp": (5, [("V", np.float32), ("Twait", np.int32)], [("W", np.float32)]),
+    # brunel_3pop_stdp without its refractory promise: the two-kernel step with deferred events
+    "brunel_3pop_stdp_deferred": (6, [("V", np.float32), ("Twait", np.int32)], [("W", np.float32)]),
 }
 
 PLASTIC = {name for name, (_, _, syn) in MODELS.items() if syn}

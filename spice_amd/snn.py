@@ -212,12 +212,15 @@ class Snn:
 
     def run_profiled(self, nsteps):
         """Runs nsteps eagerly, each kernel launched with HIP events attached to its dispatch
-        (its own duration, without the launch gap); returns summed device ms per kernel: {"receive": ..., "update": ..., "flush": ...} ("receive" includes the
-        previous step's plasticity work of plastic models, which runs in the same kernel;
-        "flush" is 0)."""
+        (its own duration, without the launch gap); returns summed device ms per kernel:
+        {"receive": ..., "update": ..., "step": ...}. Two-kernel steps: "receive" is the
+        delivery (with the previous step's plasticity work of plastic models), "update" the
+        neuron update, "step" 0. Overlapped steps: "receive" is the first delivery, "update"
+        the last update (with the passes of the step before it), "step" the nsteps - 1
+        overlapped kernels (update of step T, delivery of T + 1, plasticity work of T - 1)."""
         ms = (_c.c_double * 3)(0.0, 0.0, 0.0)
         _lib.check(_lib.lib().spice_hip_run_profiled(self._h, int(nsteps), ms))
-        return {"receive": ms[0], "update": ms[1], "flush": ms[2]}
+        return {"receive": ms[0], "update": ms[1], "step": ms[2]}
 
     def synchronize(self):
         _lib.check(_lib.lib().spice_hip_synchronize(self._h))

@@ -78,11 +78,12 @@ def test_algorithmic_bytes_model():
           "flush_written": 30}
     b = bench.algorithmic_bytes("brunel_3pop_stdp", 1000, st, 1)
     assert b["receive"] == 100 * 20 + 1000 * 8 + 10 * 4 + 200 * 4 + (50 + 30) * 4
-    assert b["flush"] == 0.0
     s = bench.algorithmic_bytes("brunel_3pop", 1000, dict(st, plastic_events=0), 1)
     assert s["receive"] == 100 * 20 + 1000 * 4
     # update: state (8 B) read + write, the V accumulator read + clear, 4 B per spike
     assert s["update"] == 1000 * (2 * 8 + 2 * 4) + 100 * 4
+    # the overlapped step kernel: one update, one delivery and one step's plasticity work
+    assert b["step"] == b["receive"] + b["update"] and s["step"] == s["receive"] + s["update"]
 
 
 def test_cpu_runnable_configuration():

@@ -154,15 +154,17 @@ def test_lazy_stdp_generic_contract_matches_dense(gpu):
         np.testing.assert_allclose(sd[k], sh[k], rtol=rtol, atol=1e-9, err_msg=k)
 
 
+@pytest.mark.parametrize("model", ["brunel_3pop_stdp", "brunel_3pop_stdp_deferred"])
 @pytest.mark.parametrize("delay", [1, 15, 255])
-def test_eager_stdp_neuron_traces_bit_exact(gpu, delay):
-    """Trace-synapse plasticity (brunel_3pop_stdp), applied eagerly -- pre pass over arriving
-    rows, post pass over spiking targets through the by-target index, events of rows arriving
-    next left to that delivery -- equals the checker's dense per-step plasticity bit for bit
-    (fixed-point mode): spikes every step, neuron state, and every weight mid-run (an export
-    applies the deferred events once and the run goes on) and at the end; delays 1, 15 and the
-    largest supported (255)."""
-    model = "brunel_3pop_stdp"
+def test_eager_stdp_neuron_traces_bit_exact(gpu, model, delay):
+    """Trace-synapse plasticity, applied eagerly -- pre pass over arriving rows, post pass over
+    spiking targets through the by-target index -- equals the checker's dense per-step
+    plasticity bit for bit (fixed-point mode): spikes every step, neuron state, and every weight
+    mid-run (an export applies the pending passes once and the run goes on) and at the end;
+    delays 1, 15 and the largest supported (255). brunel_3pop_stdp promises refractory plastic
+    endpoints, so its passes apply every event and its deliveries only read weights;
+    brunel_3pop_stdp_deferred (the same model without the promise) leaves the events of rows
+    arriving next to that delivery."""
     lay = three_pop_layout(1500)
     h = O.CpuSnn(model, lay, 1e-4, delay, seed=5, fixed_point=True)
     d = Snn(model, Layout(*lay), 1e-4, delay, seed=5, fixed_point=True)
@@ -172,11 +174,45 @@ def test_eager_stdp_neuron_traces_bit_exact(gpu, delay):
         np.testing.assert_array_equal(sd, sh, err_msg=f"step {i}: spike sets differ")
         if i % 100 == 0:
             assert_state_equal(d.neurons(), h.neurons(), f"step {i}")
-        if i in (299, 300, 601):  # twice in a row: the deferred events are applied only once
+        if i in (299, 300, 601):  # twice in a row: the pending events are applied only once
             assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), f"weights after step {i}")
     assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), "weights")
     st = d.stats()
-    assert st["plastic_events"] > 0 and st["flush_rows"] > 0 and st["flush_synapses"] > 0
+    # events applied by deliveries: only with deferred events
+    assert (st["plastic_events"] > 0) == (model == "brunel_3pop_stdp_deferred"), st
+    assert st["flush_rows"] > 0 and st["flush_synapses"] > 0
+
+
+@pytest.mark.parametrize("delay", [2, 15])
+def test_overlapped_step_equals_two_kernel_step(gpu, delay):
+    """brunel_3pop_stdp at delay >= 2 runs overlapped steps (the update of step T beside the
+    delivery of T + 1 and the passes of T - 1, one kernel per step boundary; the gate of
+    receive() tested by the update); brunel_3pop_stdp_deferred runs the two-kernel step with
+    deferred events. Same network, fixed point: identical spikes, neuron state, weights and
+    event counts after runs of assorted lengths (graph blocks, single steps, step() calls and
+    weight exports in between), and equal to the checker."""
+    lay = three_pop_layout(2500)
+    a = Snn("brunel_3pop_stdp", Layout(*lay), 1e-4, delay, seed=21, fixed_point=True)
+    b = Snn("brunel_3pop_stdp_deferred", Layout(*lay), 1e-4, delay, seed=21, fixed_point=True)
+    h = O.CpuSnn("brunel_3pop_stdp", lay, 1e-4, delay, seed=21, fixed_point=True)
+    done = 0
+    for n in (1, 2, 3, 37, 64, 5, 128, 1, 260):
+        for net in (a, b):
+            if n == 1:
+                net.step()
+            else:
+                net.run(n)
+        h.run(n)
+        done += n
+        assert_state_equal(a.neurons(), b.neurons(), f"neurons after {done} steps")
+        if n in (3, 128, 260):
+            assert_state_equal(a.synapses(ell=False), b.synapses(ell=False), f"weights after {done} steps")
+            assert_state_equal(a.synapses(ell=False), h.synapses(ell=False), f"weights vs checker after {done} steps")
+    assert_state_equal(a.neurons(), h.neurons(), "neurons vs checker")
+    sa, sb = a.stats(), b.stats()
+    assert sa["spikes"] == sb["spikes"] > 0 and sa["events"] == sb["events"], (sa, sb)
+    assert sa["plastic_events"] == 0 and sb["plastic_events"] > 0, (sa, sb)
+    a.close(), b.close(), h.close()
 
 
 def test_run_graph_equals_step(gpu):
@@ -214,15 +250,18 @@ def test_short_and_unaligned_runs_replay_graphs(gpu, model):
     assert a.stats() == b.stats()
 
 
+@pytest.mark.parametrize("model", ["brunel_3pop_stdp", "brunel_3pop_stdp_deferred"])
 @pytest.mark.parametrize("launch", [(256, 128, 1024, 256, 1), (256, 128, 1024, 512, 2), (256, 128, 1024, 1024, 1)],
                          ids=["block256", "block512", "block1024"])
-def test_run_profiled_equals_run(gpu, launch):
+def test_run_profiled_equals_run(gpu, launch, model):
     """spice_hip_run_profiled (each kernel launched with events on its dispatch) steps the net
     exactly as spice_hip_run, at each delivery workgroup size, and reports positive kernel
-    times (plastic model: plasticity inside the delivery kernel, so "flush" is 0)."""
+    times: overlapped steps (brunel_3pop_stdp at delay 15) time their first delivery, last
+    update and the overlapped kernels between; two-kernel steps (the deferred-event variant)
+    their deliveries and updates."""
     lay = three_pop_layout(2000)
-    a = Snn("brunel_3pop_stdp", Layout(*lay), 1e-4, 15, seed=9, fixed_point=True)
-    b = Snn("brunel_3pop_stdp", Layout(*lay), 1e-4, 15, seed=9, fixed_point=True)
+    a = Snn(model, Layout(*lay), 1e-4, 15, seed=9, fixed_point=True)
+    b = Snn(model, Layout(*lay), 1e-4, 15, seed=9, fixed_point=True)
     a.set_launch(*launch)
     b.set_launch(*launch)
     a.run(40)
@@ -232,7 +271,8 @@ def test_run_profiled_equals_run(gpu, launch):
     assert a.clock() == b.clock()
     assert_state_equal(a.neurons(), b.neurons(), "profiled vs run")
     assert_state_equal(a.synapses(ell=False), b.synapses(ell=False), "weights")
-    assert ms["receive"] > 0 and ms["update"] > 0 and ms["flush"] == 0
+    assert ms["receive"] > 0 and ms["update"] > 0
+    assert (ms["step"] > 0) == (model == "brunel_3pop_stdp"), ms
 
 
 def test_cpp_drop_in_api(gpu):
@@ -304,13 +344,13 @@ def test_autotune_keeps_the_network_in_place(gpu):
     assert cfg2 == cfg and us2 == us  # cached
 
 
+@pytest.mark.parametrize("model", ["brunel_3pop_stdp", "brunel_3pop_stdp_deferred"])
 @pytest.mark.parametrize("delay", [1, 15])
-def test_trace_stdp_graph_replay_bit_exact(gpu, delay):
-    """HIP-graph replay of plastic steps (the plasticity passes of each step overlapping the
-    next step on a side stream) equals the dense checker bit for bit over 1,500 steps, weights
-    exported between the graph runs, with autotuned-style launch shapes (split delivery lists,
-    small flush items)."""
-    model = "brunel_3pop_stdp"
+def test_trace_stdp_graph_replay_bit_exact(gpu, model, delay):
+    """HIP-graph replay of plastic steps (the plasticity passes of each step inside the next
+    kernel; with delay 15 brunel_3pop_stdp runs overlapped steps) equals the dense checker bit
+    for bit over 1,500 steps, weights exported between the graph runs, with autotuned-style
+    launch shapes (split delivery lists, small flush items)."""
     lay = three_pop_layout(2000)
     h = O.CpuSnn(model, lay, 1e-4, delay, seed=13, fixed_point=True)
     d = Snn(model, Layout(*lay), 1e-4, delay, seed=13, fixed_point=True)
@@ -321,7 +361,8 @@ def test_trace_stdp_graph_replay_bit_exact(gpu, delay):
         assert_state_equal(d.neurons(), h.neurons(), f"after {500 * (k + 1)} steps")
         assert_state_equal(d.synapses(ell=False), h.synapses(ell=False), f"weights after {500 * (k + 1)} steps")
     st = d.stats()
-    assert st["plastic_events"] > 0 and st["flush_rows"] > 0 and st["flush_synapses"] > 0
+    assert (st["plastic_events"] > 0) == (model == "brunel_3pop_stdp_deferred"), st
+    assert st["flush_rows"] > 0 and st["flush_synapses"] > 0
 
 
 @pytest.mark.parametrize("model", ["brunel_3pop", "brunel_3pop_stdp"])

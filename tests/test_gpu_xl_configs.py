@@ -110,7 +110,7 @@ def test_brunel_stdp_1e9(gpu):
     assert a.clock() == b.clock()
     assert_state_equal(a.neurons(), b.neurons(), "run vs step")
     st = a.stats()
-    assert st == b.stats() and st["plastic_events"] > 0 and st["flush_synapses"] > 0, st
+    assert st == b.stats() and st["flush_synapses"] > 0, st
     wa = a.synapses(ell=False)["W"]
     a.close()
     wb = b.synapses(ell=False)["W"]
