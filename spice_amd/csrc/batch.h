@@ -94,6 +94,10 @@ public:
 		_ntiles = std::int32_t( tn.size() );
 		_nblocks = std::int32_t( bn.size() );
 		_nflush = std::int32_t( fn.size() );
+		// (the overlapped kernel's update workgroups have the delivery's size: 256 threads, the
+		// batched update blocks' size)
+		_overlap = recv_block == 256;
+		for( auto * n : _nets ) _overlap = _overlap && n->overlapped();
 		upload( _tile_net, tn );
 		upload( _tile_local, tl );
 		upload( _block_net, bn );
@@ -118,10 +122,7 @@ public:
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
 		if( n == 0 ) return;
-		if( _graphs.empty() )
-			_graphs.capture( _s, [&]( int t, int len ) {
-				for( int k = 0; k < len; k++ ) launch_step( t + k );
-			} );
+		if( _graphs.empty() ) _graphs.capture( _s, [&]( int t, int len ) { launch_steps( t, len ); } );
 		_graphs.run( _s, _step, n, [&] { _step++; } );
 		for( auto * net : _nets ) net->advance_host( n );
 	}
@@ -151,6 +152,7 @@ private:
 	bool _own_stream = false;
 	step_graphs _graphs;
 	bool _fixed = false;
+	bool _overlap = false; // every net runs overlapped steps: so does the batch
 	std::int64_t _step = 0;
 	std::int32_t _ntiles = 0, _nblocks = 0, _nflush = 0;
 	int _grid_tiles = 0;
@@ -165,13 +167,46 @@ private:
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 	}
 
-	// Same schedule as hip_net::launch_step: delivery with step t - 1's plasticity, update.
-	void launch_step( std::int64_t t )
+	// Same schedules as hip_net::launch_steps: two kernels per step (delivery with step t - 1's
+	// plasticity, update), or the overlapped sequence (delivery of step t, one overlapped
+	// kernel per step boundary, the last update).
+	void launch_steps( std::int64_t t, std::int64_t len )
 	{
-		if( _fixed )
-			launch_step_t<true>( t );
-		else
-			launch_step_t<false>( t );
+		for( std::int64_t k = 0; k < len; k++ )
+		{
+			if( _overlap ) break;
+			if( _fixed )
+				launch_step_t<true>( t + k );
+			else
+				launch_step_t<false>( t + k );
+		}
+		if( !_overlap ) return;
+		for( std::int64_t k = 0; k <= len; k++ )
+		{
+			if( _fixed )
+				launch_overlap_t<true>( t, len, k );
+			else
+				launch_overlap_t<false>( t, len, k );
+		}
+	}
+
+	template <bool F>
+	void launch_overlap_t( std::int64_t t, std::int64_t len, std::int64_t k )
+	{
+		if( k == 0 )
+		{
+			// delivery of step t only: step t - 1's plasticity runs beside the first update
+			dview const * v = _views[t & ( CLOCK_SLOTS - 1 )].data();
+			SPICE_LAUNCH( k_batch_receive<Model, F, recv_block><<<unsigned( _grid_tiles ), recv_block, 0, _s>>>(
+			    v, _tile_net.data(), _tile_local.data(), _ntiles, _grid_tiles, _flush_net.data(), _flush_local.data(), _flush_count.data() ) );
+			return;
+		}
+		dview const * v = _views[( t + k - 1 ) & ( CLOCK_SLOTS - 1 )].data(); // updates step t + k - 1
+		std::int32_t const grecv = k < len ? _grid_tiles : 0, nplast = plastic ? _nflush : 0;
+		unsigned const grid = unsigned( grecv + nplast + _nblocks );
+		SPICE_LAUNCH( k_batch_step_overlap<Model, F, recv_block><<<grid, recv_block, 0, _s>>>(
+		    v, _tile_net.data(), _tile_local.data(), _ntiles, grecv, _flush_net.data(), _flush_local.data(), _flush_count.data(), nplast,
+		    _block_net.data(), _block_local.data() ) );
 	}
 
 	template <bool F>

@@ -98,6 +98,7 @@ struct dview
 	std::int32_t rparts; // delivery workgroups per tile (the step's spike list split between them)
 	std::int32_t overlap; // 1: the update of step T runs beside the delivery of T + 1 (it writes
 	                      // the spike log's step offsets then; otherwise the delivery does)
+	std::int32_t upd_slot0; // first statistics slot of the update blocks of an overlapped kernel
 	std::uint32_t flags;
 	std::uint64_t seed;
 	float param[4]; // per-net model parameters (snn_info::param)

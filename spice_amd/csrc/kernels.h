@@ -1096,6 +1096,35 @@ __global__ __launch_bounds__( BS ) void k_batch_receive( dview const * views, st
 	}
 }
 
+// Batched overlapped step (every net of the batch overlapped): k_step_overlap over the nets'
+// tables -- workgroups [0, grecv) deliver each net's next step (work items round robin),
+// [grecv, grecv + nplast) run plasticity block flush_local[b] of the flush_count[b] blocks of
+// net flush_net[b], the rest update neuron block upd_local[u] of net upd_net[u] (256 neurons).
+// grecv = 0: the last update of a run, beside the passes of the step before it.
+template <typename Model, bool FIXED, int BS>
+__global__ __launch_bounds__( BS ) void k_batch_step_overlap( dview const * views, std::int32_t const * tile_net, std::int32_t const * tile_local,
+                                                              std::int32_t ntiles, std::int32_t grecv, std::int32_t const * flush_net,
+                                                              std::int32_t const * flush_local, std::int32_t const * flush_count, std::int32_t nplast,
+                                                              std::int32_t const * upd_net, std::int32_t const * upd_local )
+{
+	std::int32_t const b = blockIdx.x;
+	if( b < grecv )
+	{
+		for( std::int32_t gt = b; gt < ntiles; gt += grecv ) receive_tile<Model, FIXED, BS, true>( views[tile_net[gt]], tile_local[gt] );
+	}
+	else if( b < grecv + nplast )
+	{
+		if constexpr( is_plastic_v<Model> )
+			plasticity_prev_block<Model, FIXED>( views[flush_net[b - grecv]], flush_local[b - grecv], flush_count[b - grecv] );
+	}
+	else
+	{
+		std::int32_t const u = b - grecv - nplast;
+		dview const & v = views[upd_net[u]];
+		update_block<Model, FIXED>( v, upd_local[u], v.upd_slot0 + upd_local[u] );
+	}
+}
+
 
 
 // Neuron update of block blk (blockDim.x <= 1024 neurons) at step T; statistics go to slot

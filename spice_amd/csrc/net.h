@@ -460,6 +460,7 @@ public:
 		apply_launch( c );
 	}
 	bool is_plastic() const override { return plastic; }
+	bool overlapped() const { return _overlap; }
 
 	net_iface * clone() const override
 	{
@@ -810,13 +811,14 @@ private:
 		_grid_tiles = std::min( v.ntiles * v.rparts, ( 16 * 1024 / c.recv_block ) * cus );
 		_grid_plast = plastic ? std::max( 1, _grid_flush * 256 / c.recv_block ) : 0; // the same waves as flush_grid
 		_lc = c;
+		v.upd_slot0 = upd_slot0();
 		// statistics slots: one per workgroup of the widest step kernel (receive: per work item;
 		// eager plasticity: after those, one per plasticity workgroup or synchronization block;
 		// the overlapped step's update blocks after those), and enough for the update kernel's
 		// blocks
 		std::size_t const slots =
 		    std::size_t( std::max( { _grid_update, _grid_upd, 2 * _grid_flush, 2 * _grid_tiles,
-		                             upd_slot0() + int( ( n + c.recv_block - 1 ) / c.recv_block ) } ) ) * NSTAT;
+		                             upd_slot0() + _grid_update } ) ) * NSTAT; // (update blocks of >= 256 neurons)
 		if( slots > _bstats.size() )
 		{
 			std::vector<unsigned long long> keep( NSTAT, 0ull );

@@ -11,7 +11,7 @@ from tests.helpers import assert_state_equal, three_pop_layout
 pytestmark = pytest.mark.gpu
 
 
-def _mixed_nets(model, count, seed0):
+def _mixed_nets(model, count, seed0, delays=(1, 2, 5, 15)):
     rng = np.random.default_rng(seed0)
     nets = []
     for k in range(count):
@@ -21,15 +21,19 @@ def _mixed_nets(model, count, seed0):
         else:
             n = int(rng.integers(300, 5000))
             lay = three_pop_layout(n)
-        delay = int(rng.choice([1, 2, 5, 15]))
+        delay = int(rng.choice(list(delays)))
         params = [float(rng.uniform(5, 30))] if model == "brunel_3pop" else None
         nets.append((Layout(*lay), delay, seed0 * 1000 + k, params))
     return nets
 
 
-@pytest.mark.parametrize("model", ["brunel_3pop", "vogels_abbott"])
-def test_batch_equals_networks_alone(gpu, model):
-    nets = _mixed_nets(model, 9, 3)
+@pytest.mark.parametrize("model,delays", [("brunel_3pop", (1, 2, 5, 15)), ("brunel_3pop", (2, 5, 15)),
+                                          ("vogels_abbott", (1, 2, 5, 15))],
+                         ids=["brunel_3pop-mixed", "brunel_3pop-overlapped", "vogels_abbott"])
+def test_batch_equals_networks_alone(gpu, model, delays):
+    """With every delay >= 2 a batch of brunel_3pop nets runs overlapped steps (one kernel per
+    step boundary over all nets); with a delay-1 net among them, two kernels per step."""
+    nets = _mixed_nets(model, 9, 3, delays)
     steps = 300  # graph replays plus eager remainder steps
     b = Batch(model, nets, 1e-4, fixed_point=True)
     b.run(steps)
@@ -47,18 +51,21 @@ def test_batch_equals_networks_alone(gpu, model):
     b.close()
 
 
-@pytest.mark.parametrize("model", ["brunel_3pop_stdp", "brunel_with_plasticity"])
-def test_plastic_batch_equals_networks_alone(gpu, model):
+@pytest.mark.parametrize("model,delays", [("brunel_3pop_stdp", (1, 4, 15)), ("brunel_3pop_stdp", (2, 4, 15)),
+                                          ("brunel_3pop_stdp_deferred", (2, 4, 15)), ("brunel_with_plasticity", (1, 4, 15))],
+                         ids=["stdp-mixed", "stdp-overlapped", "stdp_deferred", "brunel_with_plasticity"])
+def test_plastic_batch_equals_networks_alone(gpu, model, delays):
     """Plastic nets in one batch (batched receive, update and plasticity launches: eager passes
-    for brunel_3pop_stdp, lazy flushes for brunel_with_plasticity): spikes, neuron state,
-    synaptic weights and counters equal each net run alone, over enough steps for every lazy
-    row to be flushed several times (window 255 steps)."""
+    for the brunel_3pop_stdp models, lazy flushes for brunel_with_plasticity; overlapped steps
+    when every net of brunel_3pop_stdp has a delay >= 2): spikes, neuron state, synaptic
+    weights and counters equal each net run alone, over enough steps for every lazy row to be
+    flushed several times (window 255 steps)."""
     rng = np.random.default_rng(11)
     nets = []
     for k in range(5):
         n = int(rng.integers(400, 2500))
-        lay = three_pop_layout(n) if model == "brunel_3pop_stdp" else ([n // 2, n - n // 2], [(0, 1, 0.1), (1, 1, 0.1)])
-        nets.append((Layout(*lay), int(rng.choice([1, 4, 15])), 500 + k, None))
+        lay = three_pop_layout(n) if model.startswith("brunel_3pop_stdp") else ([n // 2, n - n // 2], [(0, 1, 0.1), (1, 1, 0.1)])
+        nets.append((Layout(*lay), int(rng.choice(list(delays))), 500 + k, None))
     steps = 700
     b = Batch(model, nets, 1e-4, fixed_point=True)
     b.run(steps)
@@ -79,15 +86,17 @@ def test_plastic_batch_equals_networks_alone(gpu, model):
     b.close()
 
 
-@pytest.mark.parametrize("model", ["brunel_3pop_stdp", "brunel_with_plasticity"])
-def test_plastic_batch_export_mid_run_equals_alone(gpu, model):
+@pytest.mark.parametrize("model,delays", [("brunel_3pop_stdp", (1, 4, 15)), ("brunel_3pop_stdp", (2, 4, 15)),
+                                          ("brunel_with_plasticity", (1, 4, 15))],
+                         ids=["stdp-mixed", "stdp-overlapped", "brunel_with_plasticity"])
+def test_plastic_batch_export_mid_run_equals_alone(gpu, model, delays):
     """A batched net's weights exported mid-run (the export first brings that net's synapses up
     to date: for eager plasticity it applies the last step's passes once, which the batch's next
     delivery kernel then skips for that net only), then the batch goes on: every net equals the
     same net run alone with the same exports, at both exports."""
     rng = np.random.default_rng(12)
     nets = []
-    for k, d in enumerate([1, 4, 15]):
+    for k, d in enumerate(delays):
         n = int(rng.integers(500, 1500))
         lay = three_pop_layout(n) if model == "brunel_3pop_stdp" else ([n // 2, n - n // 2], [(0, 1, 0.1), (1, 1, 0.1)])
         nets.append((Layout(*lay), d, 700 + k, None))
