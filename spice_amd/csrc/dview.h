@@ -16,6 +16,7 @@
 //   hasp     uint8[N]    row s has at least one plastic synapse
 //   pl_lo/hi int32[N]    first / one-past-last plastic slot of row s (offsets in the row)
 //   pivots   int32[N * (ntiles+1)]  per row and target tile: offset of the tile's first target
+//   tile_ids int32[nact] the tiles that some row has targets in (at least one tile)
 // lazy plasticity (per-synapse contract):
 //   last_spk int32[N]    latest spike step
 //   synced   int32[N]    row s is up to date through this step
@@ -140,6 +141,8 @@ struct dview
 
 	std::int32_t * pivots; // [N][ntiles + 1]: offset in row r of its first target >= t * tile_w
 	std::int32_t ntiles, tile_w;
+	std::int32_t * tile_ids; // [nact]: the tiles with incoming synapses (the delivery's tiles)
+	std::int32_t nact;
 
 	slog_t * slog; // nullptr: no spike log
 	std::int32_t * slog_ids;

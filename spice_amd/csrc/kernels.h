@@ -473,6 +473,16 @@ static inline __global__ __launch_bounds__( 256 ) void k_pivots( dview v )
 	}
 }
 
+// flag[t] = 1 for every tile t that some row has a target in. One wave per row.
+static inline __global__ __launch_bounds__( 256 ) void k_tile_active( dview v, std::uint8_t * flag )
+{
+	std::int64_t const r = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
+	if( r >= v.N ) return;
+	std::int32_t const * pv = v.pivots + r * ( v.ntiles + 1 );
+	for( int t = lane_id(); t < v.ntiles; t += 64 )
+		if( pv[t + 1] > pv[t] ) flag[t] = 1;
+}
+
 // Attributes of the receiving neurons that the delivering workgroup copies into LDS: those
 // that receive() declares it reads (receive_reads), of at most 4 bytes.
 template <typename Neuron, std::size_t... I>
@@ -658,10 +668,12 @@ __device__ inline void deliver_event( A &&... a )
 template <typename Model, bool FIXED, int BS, bool AHEAD = false>
 __device__ inline void receive_tile( dview const & v, std::int32_t work )
 {
-	// work item = (tile, part): the tile's spike list is split into v.rparts parts, each
-	// delivered by its own workgroup (more workgroups in flight for few, large tiles)
-	std::int32_t const tile = work / v.rparts;
-	std::int32_t const part = work - tile * v.rparts;
+	// work item = (active tile, part): the tile's spike list is split into v.rparts parts,
+	// each delivered by its own workgroup (more workgroups in flight for few, large tiles);
+	// tiles without incoming synapses get no work items
+	std::int32_t const act = work / v.rparts;
+	std::int32_t const part = work - act * v.rparts;
+	std::int32_t const tile = v.tile_ids[act];
 	using N = typename Model::neuron;
 	using S = typename Model::synapse;
 	constexpr bool plastic = is_plastic_v<Model>;
@@ -1040,7 +1052,7 @@ __global__ __launch_bounds__( BS ) void k_receive_tiled( dview v, std::int32_t g
 {
 	if( std::int32_t( blockIdx.x ) < grecv )
 	{
-		std::int32_t const nwork = v.ntiles * v.rparts;
+		std::int32_t const nwork = v.nact * v.rparts;
 		for( std::int32_t w = blockIdx.x; w < nwork; w += grecv ) receive_tile<Model, FIXED, BS>( v, w );
 	}
 	else if constexpr( is_plastic_v<Model> )
@@ -1064,7 +1076,7 @@ __global__ __launch_bounds__( BS ) void k_step_overlap( dview v, std::int32_t nr
 	std::int32_t const b = blockIdx.x;
 	if( b < nrecv )
 	{
-		std::int32_t const nwork = v.ntiles * v.rparts;
+		std::int32_t const nwork = v.nact * v.rparts;
 		for( std::int32_t w = b; w < nwork; w += nrecv ) receive_tile<Model, FIXED, BS, true>( v, w );
 	}
 	else if( b < nrecv + nplast )

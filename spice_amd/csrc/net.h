@@ -498,7 +498,7 @@ public:
 
 	// ---- used by hip_batch (batched stepping of many nets) ----
 	dview const & view( int slot ) const { return _vs[slot & ( CLOCK_SLOTS - 1 )]; }
-	int num_tiles() const { return _v.ntiles; }
+	int num_tiles() const { return _v.nact; } // the delivery's tiles (those with incoming synapses)
 	int num_receive_parts() const { return _v.rparts; }
 	int num_update_blocks() const { return _grid_update; }
 	// flush workgroups of this net inside a batch: about two per expected work item wave
@@ -552,6 +552,7 @@ private:
 	bool _overlap = false; // steps run as k_step_overlap kernels
 	launch_cfg _lc{};
 	dbuf<std::int32_t> _pivots;
+	dbuf<std::int32_t> _tile_ids;
 	std::int64_t _step = 0;
 	util::kahan_sum<float> _clock;
 	float _dt_last = 0.0f; // compensated dt of the last step (what its kernels used)
@@ -801,6 +802,8 @@ private:
 			v.tile_w = c.tile_w;
 			v.ntiles = ntiles;
 			v.pivots = _pivots.data();
+			SPICE_LAUNCH( k_pivots<<<unsigned( ( n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
+			find_active_tiles(); // (synchronizes)
 		}
 		double const mean_row = n > 0 ? double( _g.nnz ) / double( n ) : 0.0;
 		v.fparts = std::max( 1, int( mean_row / c.flush_slots + 0.5 ) );
@@ -808,7 +811,7 @@ private:
 		v.pparts = std::max( 1, int( mean_in / c.flush_slots + 0.5 ) );
 		v.rparts = c.recv_parts;
 		_grid_flush = c.flush_grid;
-		_grid_tiles = std::min( v.ntiles * v.rparts, ( 16 * 1024 / c.recv_block ) * cus );
+		_grid_tiles = std::max( 1, std::min( v.nact * v.rparts, ( 16 * 1024 / c.recv_block ) * cus ) );
 		_grid_plast = plastic ? std::max( 1, _grid_flush * 256 / c.recv_block ) : 0; // the same waves as flush_grid
 		_lc = c;
 		v.upd_slot0 = upd_slot0();
@@ -842,9 +845,30 @@ private:
 			_vs[p].ctl_next = _ctl.data() + ( ( p + 1 ) & ( CLOCK_SLOTS - 1 ) );
 			_vs[p].ctl_prev = _ctl.data() + ( ( p + CLOCK_SLOTS - 1 ) & ( CLOCK_SLOTS - 1 ) );
 		}
-		if( retile ) SPICE_LAUNCH( k_pivots<<<unsigned( ( n + 3 ) / 4 ), 256, 0, _s>>>( _v ) );
 		SPICE_HIP( hipStreamSynchronize( _s ) );
 		_graphs.clear(); // the captured launches hold the old shapes
+	}
+
+	// The delivery's tiles: those that some row has a target in (a population that no
+	// connection targets gets no delivery workgroups); at least one, so that every step has a
+	// delivery work item. Runs right after the pivots are built, before the views are copied.
+	void find_active_tiles()
+	{
+		dview & v = _v;
+		dbuf<std::uint8_t> flag( std::size_t( v.ntiles ) );
+		SPICE_HIP( hipMemsetAsync( flag.data(), 0, flag.bytes(), _s ) );
+		SPICE_LAUNCH( k_tile_active<<<unsigned( ( _g.n + 3 ) / 4 ), 256, 0, _s>>>( v, flag.data() ) );
+		std::vector<std::uint8_t> f( std::size_t( v.ntiles ) );
+		SPICE_HIP( hipMemcpyAsync( f.data(), flag.data(), f.size(), hipMemcpyDeviceToHost, _s ) );
+		SPICE_HIP( hipStreamSynchronize( _s ) );
+		std::vector<std::int32_t> ids;
+		for( std::int32_t t = 0; t < v.ntiles; t++ )
+			if( f[std::size_t( t )] ) ids.push_back( t );
+		if( ids.empty() ) ids.push_back( 0 );
+		_tile_ids.resize( ids.size() );
+		SPICE_HIP( hipMemcpy( _tile_ids.data(), ids.data(), ids.size() * 4, hipMemcpyHostToDevice ) );
+		v.tile_ids = _tile_ids.data();
+		v.nact = std::int32_t( ids.size() );
 	}
 
 	void initialize()

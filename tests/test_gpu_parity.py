@@ -67,6 +67,31 @@ def _run_pair(model, layout, delay, steps, fixed, seed=5, check_every=1, rtol=1e
     return h, d, total
 
 
+@pytest.mark.parametrize("delay", [1, 15])
+def test_tiles_without_incoming_synapses_get_no_delivery(gpu, delay):
+    """The reference's Brunel topology targets only population 1, so with 64-neuron tiles the
+    first half of the tiles has no incoming synapse and gets no delivery work items (the
+    delivery runs over the other tiles only). Fixed point, against the checker: spikes every
+    step and state bit for bit, graph-replayed (overlapped at delay 15) and single steps; and a
+    network without any synapse (every tile idle: one delivery work item remains) steps
+    exactly as the checker's."""
+    lay = brunel_ref_layout(3000)
+    h = O.CpuSnn("brunel", lay, 1e-4, delay, seed=31, fixed_point=True)
+    d = Snn("brunel", Layout(*lay), 1e-4, delay, seed=31, fixed_point=True)
+    d.set_launch(64, 0, 0, 256, 2)
+    for n in (1, 37, 128, 2, 200):
+        if n == 1:
+            np.testing.assert_array_equal(np.sort(d.step(True)), np.sort(h.step(True)))
+        else:
+            d.run(n)
+            h.run(n)
+        assert_state_equal(d.neurons(), h.neurons(), f"after {n}")
+    assert d.stats()["events"] > 0
+    d.close(), h.close()
+    empty = ([600, 600], [(0, 1, 0.0)])
+    _run_pair("brunel", empty, delay, 60, True, seed=32, check_every=20)
+
+
 @pytest.mark.parametrize("delay", [1, 8])
 def test_vogels_reference_scenario_fixed_point_bit_exact(gpu, delay):
     """Reference scenario (vogels, N=4000, p=0.02, 1000 steps, delay 1 and 8): per-step spike
