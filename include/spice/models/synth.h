@@ -15,6 +15,8 @@ struct synth : model
 			N
 		};
 
+		static constexpr std::uint32_t receive_reads = 0; // receive() reads nothing of its target
+
 		template <typename Iter, typename Backend>
 		HYBRID static void init( Iter n, snn_info, Backend & )
 		{

@@ -22,6 +22,7 @@ struct vogels_abbott : model
 			Twait
 		};
 		static constexpr std::uint32_t receive_mask = ( 1u << Gex ) | ( 1u << Gin );
+		static constexpr std::uint32_t receive_reads = 0; // receive() reads nothing of its target
 
 		template <typename Iter, typename Backend>
 		HYBRID static void init( Iter n, snn_info, Backend & )
