@@ -599,6 +599,24 @@ def main():
             "l2_atomic_requests_per_s": round(counted["l2_atomic_requests"] / (avg_ms * 1e-3), 1),
             "source": "profiles/pmc_traffic.json"}
 
+    # The passes' traffic is random 64-byte lines, whose rate -- not the streaming peak -- bounds
+    # the plastic step: the same kernel's line transfers per second next to the device's rate for
+    # random 4-byte loads (one line in) and read-modify-writes (one line in, one out) in an array
+    # the size of the plastic weights (~1.6 GB at 1e9 synapses)
+    if model.startswith("brunel_3pop_stdp"):
+        from spice_amd.snn import probe_random_lines
+        acc = 1 << 21
+        nbytes = 1_600_000_000
+        t_rd = probe_random_lines(nbytes, acc, False)
+        t_rmw = probe_random_lines(nbytes, acc, True)
+        rl = {"probe_bytes": nbytes, "probe_accesses": acc,
+              "probe_read_lines_per_s": round(acc / (t_rd * 1e-6), 1),
+              "probe_rmw_line_transfers_per_s": round(2 * acc / (t_rmw * 1e-6), 1)}
+        if traffic:
+            rl["kernel_line_transfers_per_s"] = round(traffic / 64 / (avg_ms * 1e-3), 1)
+            rl["kernel_frac_of_rmw_probe"] = round(rl["kernel_line_transfers_per_s"] / rl["probe_rmw_line_transfers_per_s"], 4)
+        roofline["random_lines"] = rl
+
     cpu = None
     if args.cpu_steps > 0:
         s_per_step, t_init, core = cpu_baseline(model, (pops, conns), delay, args.seed, net, args.cpu_steps)

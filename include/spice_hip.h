@@ -421,6 +421,14 @@ int spice_hip_set_clock( spice_net * net, int64_t step, float sum, float c );
 int spice_hip_get_stats( spice_net * net, spice_stats * out );
 int spice_hip_reset_stats( spice_net * net );
 
+/* No counterpart in the reference: a measurement of the device, not of a network. Times
+ * `accesses` 4-byte accesses at pseudo-random places of a fresh `bytes`-byte device array (a
+ * load each, and with rmw != 0 the store of the changed value after it: the plasticity post
+ * pass's access to a weight), averaged over `reps` launches after one untimed launch; returns
+ * the microseconds per launch. Allocates and frees the array (synchronous). The rate of random
+ * 64-byte memory lines it yields bounds the kernels whose traffic is such lines. */
+int spice_hip_probe_random_lines( uint64_t bytes, uint64_t accesses, int32_t rmw, int32_t reps, double * out_us );
+
 /* ---- state export / import (synchronous) ------------------------------------------------ */
 
 /* adj(): ELL edges (num_neurons x width, INT_MAX-padded) (spice/snn.h:32,

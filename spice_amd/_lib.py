@@ -101,6 +101,7 @@ SIGNATURES = {
     "spice_hip_get_launch": (ctypes.c_int, [_P, ctypes.POINTER(Launch)]),
     "spice_hip_set_launch": (ctypes.c_int, [_P, ctypes.POINTER(Launch)]),
     "spice_hip_autotune": (ctypes.c_int, [_P, _I64, _I64, ctypes.POINTER(Launch), ctypes.POINTER(ctypes.c_double)]),
+    "spice_hip_probe_random_lines": (ctypes.c_int, [_U64, _U64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "spice_server_create": (ctypes.c_int, [_U64, _I64, SPIKE_SINK, _P, ctypes.POINTER(_P)]),
     "spice_server_destroy": (ctypes.c_int, [_P]),
     "spice_server_estimate": (ctypes.c_int, [ctypes.POINTER(JobDesc), _I64, _PU64]),

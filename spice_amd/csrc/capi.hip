@@ -496,6 +496,82 @@ int spice_hip_reset_stats( spice_net * net )
 {
 	return guard( [&] { impl( net )->reset_stats(); } );
 }
+
+namespace
+{
+// SplitMix64 output function (S. Vigna's published finalizer): a pseudo-random 64-bit value
+// per access index.
+__device__ inline std::uint64_t mix64( std::uint64_t z )
+{
+	z = ( z ^ ( z >> 30 ) ) * 0xbf58476d1ce4e5b9ull;
+	z = ( z ^ ( z >> 27 ) ) * 0x94d049bb133111ebull;
+	return z ^ ( z >> 31 );
+}
+
+// `count` 4-byte accesses at pseudo-random places of a[0 .. n): a load, and with rmw the store
+// of the changed value after it -- the plasticity post pass's access to a weight, without its
+// index and trace gathers. 4 accesses per thread in flight.
+__global__ __launch_bounds__( 256 ) void k_probe_lines( float * a, std::uint64_t n, std::uint64_t count, std::uint64_t seed, int rmw )
+{
+	constexpr int U = 4;
+	std::uint64_t const nthreads = std::uint64_t( gridDim.x ) * blockDim.x;
+	float sink = 0.0f;
+	for( std::uint64_t i0 = std::uint64_t( blockIdx.x ) * blockDim.x + threadIdx.x; i0 < count; i0 += U * nthreads )
+	{
+		std::uint64_t idx[U];
+		float x[U];
+#pragma unroll
+		for( int u = 0; u < U; u++ ) idx[u] = mix64( i0 + u * nthreads + seed ) % n;
+#pragma unroll
+		for( int u = 0; u < U; u++ ) x[u] = i0 + u * nthreads < count ? a[idx[u]] : 0.0f;
+#pragma unroll
+		for( int u = 0; u < U; u++ )
+		{
+			if( i0 + u * nthreads >= count ) continue;
+			if( rmw )
+				a[idx[u]] = x[u] + 1.0f;
+			else
+				sink += x[u];
+		}
+	}
+	if( sink < 0.0f ) a[0] = sink; // never (the array holds non-negative values): keeps the loads
+}
+} // namespace
+
+int spice_hip_probe_random_lines( uint64_t bytes, uint64_t accesses, int32_t rmw, int32_t reps, double * out_us )
+{
+	return guard( [&] {
+		require( out_us != nullptr, "null output" );
+		require( bytes >= 64 && bytes <= ( std::uint64_t( 64 ) << 30 ), "bytes must be in [64, 64 GiB]" );
+		require( accesses >= 1 && accesses <= ( std::uint64_t( 1 ) << 32 ) && reps >= 1 && reps <= 10000, "invalid access or repetition count" );
+		std::uint64_t const n = bytes / 4;
+		dbuf<float> a( n );
+		hipEvent_t e0 = nullptr, e1 = nullptr;
+		try
+		{
+			SPICE_HIP( hipMemset( a.data(), 0, a.bytes() ) );
+			SPICE_HIP( hipEventCreate( &e0 ) );
+			SPICE_HIP( hipEventCreate( &e1 ) );
+			unsigned const grid = unsigned( 8 * device_cu_count() );
+			SPICE_LAUNCH( k_probe_lines<<<grid, 256>>>( a.data(), n, accesses, 0, rmw ) ); // warm
+			SPICE_HIP( hipEventRecord( e0, nullptr ) );
+			for( int r = 0; r < reps; r++ ) SPICE_LAUNCH( k_probe_lines<<<grid, 256>>>( a.data(), n, accesses, std::uint64_t( r + 1 ) << 40, rmw ) );
+			SPICE_HIP( hipEventRecord( e1, nullptr ) );
+			SPICE_HIP( hipEventSynchronize( e1 ) );
+			float ms = 0.0f;
+			SPICE_HIP( hipEventElapsedTime( &ms, e0, e1 ) );
+			*out_us = double( ms ) * 1e3 / reps;
+		}
+		catch( ... )
+		{
+			if( e0 ) (void)hipEventDestroy( e0 );
+			if( e1 ) (void)hipEventDestroy( e1 );
+			throw;
+		}
+		(void)hipEventDestroy( e0 );
+		(void)hipEventDestroy( e1 );
+	} );
+}
 int spice_hip_export_adj( spice_net * net, int32_t * out, uint64_t cap )
 {
 	return guard( [&] {

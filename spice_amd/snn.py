@@ -86,6 +86,16 @@ def generate_graph(layout, seed, d_row_ptr, d_col=None, col_cap=0, noalloc=False
     return nnz.value, width.value
 
 
+def probe_random_lines(nbytes, accesses, rmw, reps=20):
+    """Microseconds per launch of `accesses` 4-byte accesses at pseudo-random places of a fresh
+    device array of `nbytes` bytes (a load each; with rmw, the store of the changed value after
+    it), averaged over `reps` launches: the device's rate of random memory lines
+    (spice_hip_probe_random_lines)."""
+    us = _c.c_double()
+    _lib.check(_lib.lib().spice_hip_probe_random_lines(int(nbytes), int(accesses), 1 if rmw else 0, int(reps), _c.byref(us)))
+    return us.value
+
+
 def _desc(layout, delay, seed, params, keep):
     """NetDesc for a layout; `keep` collects the ctypes buffers the descriptor points to."""
     if not isinstance(layout, Layout):

@@ -427,3 +427,17 @@ def test_generate_graph_into_device_buffers(gpu, name, layout, seed):
     if nnz > 1:
         with pytest.raises(ValueError, match="column buffer"):
             generate_graph(layout, seed, rp.data_ptr(), col.data_ptr(), nnz - 1)
+
+
+def test_probe_random_lines(gpu):
+    """The random-line probe (spice_hip_probe_random_lines) times reads and read-modify-writes
+    of 4-byte values at pseudo-random places: positive times, a write-back costs time, and
+    invalid sizes are rejected before any device work."""
+    from spice_amd.snn import probe_random_lines
+    rd = probe_random_lines(1 << 28, 1 << 20, False, reps=5)
+    rmw = probe_random_lines(1 << 28, 1 << 20, True, reps=5)
+    assert rd > 0 and rmw > 0
+    assert rmw > 0.5 * rd
+    for args in ((32, 1 << 20, False), (1 << 28, 0, False), (1 << 40, 1, True)):
+        with pytest.raises(ValueError):
+            probe_random_lines(*args, reps=1)
