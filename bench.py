@@ -504,10 +504,7 @@ def main():
                     help="timed steps (100,000 = 10 s bio: the configuration's and the reference's benchmark length)")
     ap.add_argument("--warmup", type=int, default=2_000)
     ap.add_argument("--workload", default="brunel_stdp_1e9", choices=sorted(WORKLOADS))
-    # the profile pass (per-kernel times for the roofline) runs between warmup and timed steps:
-    # one second of biological time, so that a short timed window samples the network after its
-    # first-second population bursts, as most of a 10 s run does
-    ap.add_argument("--profile-steps", type=int, default=10_000)
+    ap.add_argument("--profile-steps", type=int, default=1_000)
     ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU baseline (0 = skip); ~0.55 s each at 1e9 synapses")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--flags", type=int, default=0, help="spice_flag bits (1: int64 fixed-point accumulation; see include/spice_hip.h)")
