@@ -1,7 +1,8 @@
 #pragma once
 // Model-templated step kernels for gfx950 (subsystems b, c, d).
 //
-// Per step T (step index read from device memory, see dview.h), two kernels on one stream:
+// Per step T (step index read from device memory, see dview.h), two kernels on one stream, or
+// (overlapped steps) one kernel per step boundary that holds the work of both:
 //   k_receive_tiled<M> : delivers the spikes emitted at T - D. One workgroup per (target tile,
 //                  part of the step's spike list): the spikes' row segments inside the tile
 //                  are walked as one flat index space and M::neuron::receive accumulates
@@ -18,6 +19,11 @@
 //                  plasticity work. Block 0 writes step T + 1's clock and empties its counters,
 //                  so the same launches are valid at every step and are captured once into a
 //                  HIP graph.
+//   k_step_overlap<M>: for models whose delivery may run before its targets' update (receive()
+//                  split into a gate the update tests, or reading nothing of the target; for
+//                  plastic models, eager with refractory endpoints) and a delay of at least 2:
+//                  the update of step T, the delivery of step T + 1 and the plasticity passes
+//                  of step T - 1 in one kernel, one kernel per step boundary.
 //
 // Reference counterparts: _process_neurons (spice/cuda/algorithm.cu:181-230),
 // _process_spikes HNDL_SPKS / UPDT_SYNS (algorithm.cu:238-313), _gather (algorithm.cu:340-384),
