@@ -439,8 +439,14 @@ __global__ __launch_bounds__( 256 ) void k_init_synapses( dview v )
 #ifndef SPICE_UPDATE_BLOCK
 #define SPICE_UPDATE_BLOCK 256 // threads per neuron-update workgroup of a single net (64..256)
 #endif
-static_assert( SPICE_UPDATE_BLOCK % 64 == 0 && SPICE_UPDATE_BLOCK >= 64 && SPICE_UPDATE_BLOCK <= 256,
-               "SPICE_UPDATE_BLOCK must be 64, 128, 192 or 256 threads" );
+// update_block's per-wave compaction counters hold at most 16 waves
+static_assert( SPICE_UPDATE_BLOCK % 64 == 0 && SPICE_UPDATE_BLOCK >= 64 && SPICE_UPDATE_BLOCK <= 1024,
+               "SPICE_UPDATE_BLOCK must be a multiple of 64 threads, at most 1024" );
+// Large networks update in workgroups of this many threads: each workgroup reserves its
+// spikes' ring slots with one atomic on the step's counter, and at N = 800,640 (the reference's
+// synth network) 3,128 workgroups of 256 serialized on it (29 us per update; 12.5 us with 1024).
+constexpr int UPDATE_BLOCK_LARGE = 1024;
+constexpr std::int64_t UPDATE_LARGE_N = 131072; // from this many neurons
 #ifndef SPICE_BATCH_ITEMS_PER_CU
 #define SPICE_BATCH_ITEMS_PER_CU 64
 #endif
@@ -1395,8 +1401,8 @@ __device__ inline void update_block( dview const & v, std::int64_t blk, std::int
 	}
 }
 
-template <typename Model, bool FIXED>
-__global__ __launch_bounds__( SPICE_UPDATE_BLOCK ) void k_update( dview v )
+template <typename Model, bool FIXED, int BS = SPICE_UPDATE_BLOCK>
+__global__ __launch_bounds__( BS ) void k_update( dview v )
 {
 	update_block<Model, FIXED>( v, blockIdx.x, blockIdx.x );
 }

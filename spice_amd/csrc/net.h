@@ -547,7 +547,8 @@ private:
 	dview _vs[CLOCK_SLOTS]{};
 
 	int _grid_update = 0, _grid_flush = 0, _grid_tiles = 0;
-	int _grid_upd = 0; // workgroups of this net's own update kernel (SPICE_UPDATE_BLOCK threads)
+	int _grid_upd = 0; // workgroups of this net's own update kernel (_upd_block threads)
+	int _upd_block = SPICE_UPDATE_BLOCK;
 	int _grid_plast = 0; // plasticity workgroups of the delivery kernel
 	bool _overlap = false; // steps run as k_step_overlap kernels
 	launch_cfg _lc{};
@@ -686,7 +687,8 @@ private:
 
 		if constexpr( traced ) build_by_target();
 		_grid_update = int( ( n + 255 ) / 256 );
-		_grid_upd = int( ( n + SPICE_UPDATE_BLOCK - 1 ) / SPICE_UPDATE_BLOCK );
+		_upd_block = n >= UPDATE_LARGE_N ? UPDATE_BLOCK_LARGE : SPICE_UPDATE_BLOCK;
+		_grid_upd = int( ( n + _upd_block - 1 ) / _upd_block );
 		apply_launch( launch_cfg{} );
 	}
 
@@ -988,12 +990,33 @@ private:
 		if( fixed() )
 		{
 			launch_receive<true>( v );
-			SPICE_LAUNCH( k_update<Model, true><<<_grid_upd, SPICE_UPDATE_BLOCK, 0, _s>>>( v ) );
+			launch_update<true>( v, nullptr, nullptr );
 		}
 		else
 		{
 			launch_receive<false>( v );
-			SPICE_LAUNCH( k_update<Model, false><<<_grid_upd, SPICE_UPDATE_BLOCK, 0, _s>>>( v ) );
+			launch_update<false>( v, nullptr, nullptr );
+		}
+	}
+
+	// the neuron update kernel of the two-kernel step (workgroups of _upd_block threads), with
+	// HIP events attached to its dispatch when start is not null
+	template <bool F>
+	void launch_update( dview const & v, hipEvent_t start, hipEvent_t stop )
+	{
+		if( _upd_block == UPDATE_BLOCK_LARGE )
+		{
+			if( start )
+				launch_timed( reinterpret_cast<void const *>( &k_update<Model, F, UPDATE_BLOCK_LARGE> ), unsigned( _grid_upd ), UPDATE_BLOCK_LARGE, start, stop, v );
+			else
+				SPICE_LAUNCH( ( k_update<Model, F, UPDATE_BLOCK_LARGE><<<_grid_upd, UPDATE_BLOCK_LARGE, 0, _s>>>( v ) ) );
+		}
+		else
+		{
+			if( start )
+				launch_timed( reinterpret_cast<void const *>( &k_update<Model, F> ), unsigned( _grid_upd ), SPICE_UPDATE_BLOCK, start, stop, v );
+			else
+				SPICE_LAUNCH( ( k_update<Model, F><<<_grid_upd, SPICE_UPDATE_BLOCK, 0, _s>>>( v ) ) );
 		}
 	}
 
@@ -1032,7 +1055,7 @@ private:
 			launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, 512> ), grid, 512, e[0], e[1], v, tiles );
 		else
 			launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, 1024> ), grid, 1024, e[0], e[1], v, tiles );
-		launch_timed( reinterpret_cast<void const *>( &k_update<Model, F> ), unsigned( _grid_upd ), SPICE_UPDATE_BLOCK, e[2], e[3], v );
+		launch_update<F>( v, e[2], e[3] );
 	}
 
 	// Brings all plastic synapses up to date through the last completed step.

@@ -441,3 +441,23 @@ def test_probe_random_lines(gpu):
     for args in ((32, 1 << 20, False), (1 << 28, 0, False), (1 << 40, 1, True)):
         with pytest.raises(ValueError):
             probe_random_lines(*args, reps=1)
+
+
+@pytest.mark.parametrize("model", ["synth", "vogels_abbott"])
+def test_large_network_update_blocks_bit_exact(gpu, model):
+    """From 131,072 neurons the two-kernel step updates in 1024-thread workgroups (fewer of
+    them reserve ring slots on the step's spike counter). Delay 1 (two kernels per step), N =
+    150,000, fixed point, against the checker: spikes every single step and state bit for bit
+    after single and graph-replayed steps (Vogels-Abbott: the whole population's first,
+    synchronous spike within them)."""
+    n = 150_000
+    lay = ([n], [(0, 0, 0.0002)])
+    h = O.CpuSnn(model, lay, 1e-4, 1, seed=51, fixed_point=True)
+    d = Snn(model, Layout(*lay), 1e-4, 1, seed=51, fixed_point=True)
+    for i in range(20):
+        np.testing.assert_array_equal(np.sort(d.step(True)), np.sort(h.step(True)), err_msg=f"step {i}")
+    d.run(280)  # Vogels-Abbott's background current brings every neuron to threshold at ~140 steps
+    h.run(280)
+    assert_state_equal(d.neurons(), h.neurons(), "after 300 steps")
+    assert d.stats()["spikes"] > 0
+    d.close(), h.close()
