@@ -57,9 +57,9 @@ public:
 		{
 			launch_cfg c = n->get_launch();
 			int const tw = int( std::clamp<std::int64_t>( ( n->num_neurons() / tiles_per_net + 63 ) / 64 * 64, 64, TILE_MAX ) );
-			if( tw > c.tile_w )
+			if( tw > c.tile_w || c.tile_w > TILE_MAX ) // (the batched kernels hold tiles of up to TILE_MAX)
 			{
-				c.tile_w = tw;
+				c.tile_w = c.tile_w > TILE_MAX ? TILE_MAX : tw;
 				n->set_launch( c );
 			}
 		}

@@ -836,11 +836,12 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			};
 			// delivery: tile width x workgroup size, then the split of each tile's spikes
 			for( int bs : { 256, 512, 1024 } )
-				for( int tw : { 128, 256, 512, 1024 } )
+				for( int tw : { 128, 256, 512, 1024, 2048, 4096 } )
 				{
 					std::int64_t const tiles = ( N + tw - 1 ) / tw;
-					if( ( tw == start.tile_w && bs == start.recv_block ) || N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) )
-						continue; // the start shape, or pivots above 4 GB
+					if( ( tw == start.tile_w && bs == start.recv_block ) || N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) ||
+					    tw > n->max_tile_w() )
+						continue; // the start shape, pivots above 4 GB, or wider than the net's tiles
 					launch_cfg x = start;
 					x.tile_w = tw;
 					x.recv_block = bs;

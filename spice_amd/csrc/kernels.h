@@ -457,6 +457,11 @@ constexpr std::int64_t UPDATE_LARGE_N = 131072; // from this many neurons
 #define SPICE_BATCH_RECV_BLOCK 256
 #endif
 constexpr int TILE_MAX = 1024; // targets per tile (LDS accumulators)
+// Wide tiles (static models, two-kernel step only: a second instantiation of the delivery kernel,
+// so the other kernels keep the smaller LDS footprint). A sparse network's rows have few targets
+// in each 1,024-neuron tile (the reference's synth network: ~1.6), and every delivery workgroup
+// stages all of the step's spikes: wider tiles stage each spike in fewer workgroups.
+constexpr int TILE_WIDE = 4096;
 // Delivery workgroups have 256 or 1024 threads (a launch shape): they stage that many spikes
 // per pass in LDS.
 
@@ -544,14 +549,14 @@ constexpr bool acc_needs_8( std::index_sequence<I...> )
 template <typename Neuron, bool FIXED>
 using acc_word = std::conditional_t<acc_needs_8<Neuron, FIXED>( std::make_index_sequence<std::size_t( Neuron::size )>{} ), unsigned long long, std::uint32_t>;
 
-template <typename Neuron, bool FIXED>
+template <typename Neuron, bool FIXED, int TM = TILE_MAX>
 struct tile_iter
 {
 	dview const * v;
 	std::int32_t j;
 	std::int32_t t0;
-	acc_word<Neuron, FIXED> * lds; // [nacc][TILE_MAX] accumulator slots
-	std::uint32_t const * stage;   // [nstaged][TILE_MAX] 4-byte slots
+	acc_word<Neuron, FIXED> * lds; // [nacc][TM] accumulator slots
+	std::uint32_t const * stage;   // [nstaged][TM] 4-byte slots
 	template <int I>
 	__device__ auto get() const
 	{
@@ -563,15 +568,15 @@ struct tile_iter
 		using A = acc_type<T, FIXED>;
 		constexpr bool staged = ( ( smask >> I ) & 1u ) != 0;
 		std::uint32_t sv = 0;
-		if constexpr( staged ) sv = stage[b * TILE_MAX + ( j - t0 )];
-		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, staged>{ v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0, sv };
+		if constexpr( staged ) sv = stage[b * TM + ( j - t0 )];
+		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, staged>{ v, j, reinterpret_cast<A *>( lds + a * TM ), t0, sv };
 	}
 	__device__ uint_ id() const { return uint_( j ); }
 };
 
 // The same target, with the staged attribute values already read (sval[k]: the k-th staged
 // attribute): the delivery of static models reads them for all entries of a pass at once.
-template <typename Neuron, bool FIXED>
+template <typename Neuron, bool FIXED, int TM = TILE_MAX>
 struct tile_iter_vals
 {
 	static constexpr int NS = __builtin_popcount( staged_mask<Neuron>() ) > 0 ? __builtin_popcount( staged_mask<Neuron>() ) : 1;
@@ -590,7 +595,7 @@ struct tile_iter_vals
 		constexpr int b = __builtin_popcount( smask & ( ( 1u << I ) - 1u ) );
 		using A = acc_type<T, FIXED>;
 		constexpr bool staged = ( ( smask >> I ) & 1u ) != 0;
-		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, staged>{ v, j, reinterpret_cast<A *>( lds + a * TILE_MAX ), t0,
+		return tile_proxy<T, I, A, ( ( mask >> I ) & 1u ) != 0, staged>{ v, j, reinterpret_cast<A *>( lds + a * TM ), t0,
 		                                                                  staged ? sval[b] : 0u };
 	}
 	__device__ uint_ id() const { return uint_( j ); }
@@ -677,7 +682,7 @@ __device__ inline void deliver_event( A &&... a )
 // LDS with LDS atomics; at the end the tile's sums go to the global accumulators with plain
 // stores: one workgroup owns each tile, so no global atomics are needed. AHEAD: the kernel's
 // clock is step T - 1's (the overlapped step kernel delivers the next step beside the update).
-template <typename Model, bool FIXED, int BS, bool AHEAD = false>
+template <typename Model, bool FIXED, int BS, bool AHEAD = false, int TM = TILE_MAX>
 __device__ inline void receive_tile( dview const & v, std::int32_t work )
 {
 	// work item = (active tile, part): the tile's spike list is split into v.rparts parts,
@@ -696,8 +701,8 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	constexpr std::uint32_t smask = gated_receive_v<N> ? 0u : staged_mask<N>();
 	constexpr int nstaged = __builtin_popcount( smask );
 	using word = acc_word<N, FIXED>;
-	__shared__ word acc[nacc * TILE_MAX];
-	__shared__ std::uint32_t stage[nstaged > 0 ? nstaged * TILE_MAX : 1];
+	__shared__ word acc[nacc * TM];
+	__shared__ std::uint32_t stage[nstaged > 0 ? nstaged * TM : 1];
 	// a chunk's non-empty segments (spike rows inside the tile), compacted: one record (row-store
 	// offset, source) read per event, the first flat event index, and for finding an event's
 	// segment without a search (chunks of at most 32 * SW events) a bitmap of the flat indices
@@ -722,8 +727,8 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	__shared__ std::int32_t seg_synced[lazy ? BS : 1];
 	__shared__ std::int32_t seg_pend[lazy || deferred ? BS : 1]; // deferred: the row's spike also arrived at T - 1
 	__shared__ float seg_xf[deferred ? BS : 1];                  // deferred: the row's pre trace after T - 1
-	__shared__ std::uint8_t tile_f8[deferred ? TILE_MAX : 1];    // deferred: the tile's step T - 1 flags
-	__shared__ float tile_y[deferred ? TILE_MAX : 1];            // deferred: the tile's post traces after T - 1
+	__shared__ std::uint8_t tile_f8[deferred ? TM : 1];    // deferred: the tile's step T - 1 flags
+	__shared__ float tile_y[deferred ? TM : 1];            // deferred: the tile's post traces after T - 1
 	__shared__ std::uint64_t wsum[BS / 64];
 
 	std::int64_t const T = v.ctl->step + ( AHEAD ? 1 : 0 );
@@ -751,7 +756,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	{
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
-		for( int x = tid; x < nacc * TILE_MAX; x += BS ) acc[x] = word( 0 );
+		for( int x = tid; x < nacc * TM; x += BS ) acc[x] = word( 0 );
 		if constexpr( deferred )
 			if( catch_up ) // what step T - 1's deferred events read of their targets
 				for( int x = tid; x < tw; x += BS )
@@ -767,7 +772,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 				using T = typename N::template type<I>;
 				using B = std::conditional_t<sizeof( T ) == 1, std::uint8_t, std::conditional_t<sizeof( T ) == 2, std::uint16_t, std::uint32_t>>;
 				constexpr int b = __builtin_popcount( smask & ( ( 1u << I ) - 1u ) );
-				for( int x = tid; x < tw; x += BS ) stage[b * TILE_MAX + x] = std::uint32_t( __builtin_bit_cast( B, ld<T>( v.na[I], t0 + x ) ) );
+				for( int x = tid; x < tw; x += BS ) stage[b * TM + x] = std::uint32_t( __builtin_bit_cast( B, ld<T>( v.na[I], t0 + x ) ) );
 			}
 		} );
 		__syncthreads();
@@ -913,7 +918,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 						if( !ok[u] ) continue;
 						std::int32_t const s = sv[u];
 						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
-						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc, stage };
+						tile_iter<N, FIXED, TM> dst{ &v, j[u], t0, acc, stage };
 						reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
 						bool const pre = ( f8[u] & 0x80 ) != 0, post = ( f8[u] & F8_SPIKED ) != 0;
 						if( pre || post )
@@ -942,7 +947,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 						if( !ok[u] ) continue;
 						std::int32_t const s = sv[u];
 						tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
-						tile_iter<N, FIXED> dst{ &v, j[u], t0, acc, stage };
+						tile_iter<N, FIXED, TM> dst{ &v, j[u], t0, acc, stage };
 						reg_iter<typename S::tuple_t> itr{ sy[u], std::uint32_t( e[u] ) };
 						deliver_event<N>( s, dst, itr, info, bak );
 					}
@@ -951,13 +956,13 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 				{
 					// the staged attribute values of all U targets first (one LDS round), then
 					// receive() for each entry
-					using iter_v = tile_iter_vals<N, FIXED>;
+					using iter_v = tile_iter_vals<N, FIXED, TM>;
 					std::uint32_t svals[U][iter_v::NS];
 #pragma unroll
 					for( int u = 0; u < U; u++ )
 #pragma unroll
 						for( int b = 0; b < iter_v::NS; b++ )
-							svals[u][b] = nstaged > 0 ? stage[b * TILE_MAX + ( ok[u] ? j[u] - t0 : 0 )] : 0u;
+							svals[u][b] = nstaged > 0 ? stage[b * TM + ( ok[u] ? j[u] - t0 : 0 )] : 0u;
 #pragma unroll
 					for( int u = 0; u < U; u++ )
 					{
@@ -978,7 +983,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 					if( !ok[u] ) continue;
 					std::int32_t const s = sv[u];
 					tile_backend<FIXED> bak{ make_backend<FIXED>( v, std::uint32_t( s ), std::uint32_t( j[u] ), std::uint32_t( T ), util::TAG_RECEIVE ) };
-					tile_iter<N, FIXED> dst{ &v, j[u], t0, acc, stage };
+					tile_iter<N, FIXED, TM> dst{ &v, j[u], t0, acc, stage };
 					if constexpr( plastic )
 					{
 						typename S::tuple_t sy;
@@ -1015,7 +1020,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 			{
 				using A = acc_type<typename N::template type<I>, FIXED>;
 				constexpr int a = __builtin_popcount( rmask & ( ( 1u << I ) - 1u ) );
-				A const * src = reinterpret_cast<A const *>( acc + a * TILE_MAX );
+				A const * src = reinterpret_cast<A const *>( acc + a * TM );
 				for( int x = tid; x < tw; x += BS )
 				{
 					A const val = src[x];
@@ -1059,13 +1064,13 @@ __device__ inline void plasticity_prev_block( dview const & v, std::int64_t blk,
 
 // Workgroups [0, grecv) deliver (work items round robin); plastic models give the workgroups
 // [grecv, gridDim.x) to step T - 1's plasticity work.
-template <typename Model, bool FIXED, int BS>
+template <typename Model, bool FIXED, int BS, int TM = TILE_MAX>
 __global__ __launch_bounds__( BS ) void k_receive_tiled( dview v, std::int32_t grecv )
 {
 	if( std::int32_t( blockIdx.x ) < grecv )
 	{
 		std::int32_t const nwork = v.nact * v.rparts;
-		for( std::int32_t w = blockIdx.x; w < nwork; w += grecv ) receive_tile<Model, FIXED, BS>( v, w );
+		for( std::int32_t w = blockIdx.x; w < nwork; w += grecv ) receive_tile<Model, FIXED, BS, false, TM>( v, w );
 	}
 	else if constexpr( is_plastic_v<Model> )
 		plasticity_prev_block<Model, FIXED>( v, blockIdx.x - grecv, gridDim.x - grecv );

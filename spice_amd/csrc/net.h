@@ -86,6 +86,7 @@ struct net_iface
 	virtual void set_launch( launch_cfg c ) = 0;
 	virtual net_iface * clone() const = 0;
 	virtual bool is_plastic() const = 0;
+	virtual int max_tile_w() const = 0; // widest tile_w set_launch accepts
 };
 
 inline int device_cu_count()
@@ -461,6 +462,8 @@ public:
 	}
 	bool is_plastic() const override { return plastic; }
 	bool overlapped() const { return _overlap; }
+	// widest delivery tile: wide tiles for the two-kernel step of static models
+	int max_tile_w() const override { return !plastic && !_overlap ? TILE_WIDE : TILE_MAX; }
 
 	net_iface * clone() const override
 	{
@@ -789,8 +792,9 @@ private:
 			throw std::invalid_argument( "recv_block must be 256, 512 or 1024" );
 		if( c.recv_parts < 1 || c.recv_parts > 16 ) throw std::invalid_argument( "recv_parts must be in [1, 16]" );
 		int const cus = device_cu_count();
-		if( c.tile_w < 64 || c.tile_w > TILE_MAX || c.tile_w % 64 )
-			throw std::invalid_argument( "tile_w must be a multiple of 64 in [64, 1024]" );
+		if( c.tile_w < 64 || c.tile_w > max_tile_w() || c.tile_w % 64 )
+			throw std::invalid_argument( max_tile_w() > TILE_MAX ? "tile_w must be a multiple of 64 in [64, 4096]"
+			                                                    : "tile_w must be a multiple of 64 in [64, 1024] (wider tiles: static models with a delay of 1)" );
 		if( c.flush_slots < 64 || c.flush_slots > ( 1 << 20 ) || c.flush_slots % 64 )
 			throw std::invalid_argument( "flush_slots must be a multiple of 64 in [64, 2^20]" );
 		if( c.flush_grid < 1 || c.flush_grid > 64 * cus ) throw std::invalid_argument( "flush_grid must be in [1, 64 x CUs]" );
@@ -1024,13 +1028,38 @@ private:
 	template <bool F>
 	void launch_receive( dview const & v )
 	{
+		launch_receive_events<F>( v, nullptr, nullptr );
+	}
+
+	// the delivery kernel at the launch shape's workgroup size and tile capacity (wide tiles:
+	// static models only), with HIP events attached to its dispatch when start is not null
+	template <bool F>
+	void launch_receive_events( dview const & v, hipEvent_t start, hipEvent_t stop )
+	{
 		unsigned const grid = unsigned( _grid_tiles + _grid_plast );
-		if( _lc.recv_block == 256 )
-			SPICE_LAUNCH( k_receive_tiled<Model, F, 256><<<grid, 256, 0, _s>>>( v, _grid_tiles ) );
-		else if( _lc.recv_block == 512 )
-			SPICE_LAUNCH( k_receive_tiled<Model, F, 512><<<grid, 512, 0, _s>>>( v, _grid_tiles ) );
-		else
-			SPICE_LAUNCH( k_receive_tiled<Model, F, 1024><<<grid, 1024, 0, _s>>>( v, _grid_tiles ) );
+		std::int32_t const tiles = _grid_tiles;
+		auto go = [&]( auto bs, auto tm ) {
+			constexpr int BS = decltype( bs )::value, TM = decltype( tm )::value;
+			if( start )
+				launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, BS, TM> ), grid, BS, start, stop, v, tiles );
+			else
+				SPICE_LAUNCH( ( k_receive_tiled<Model, F, BS, TM><<<grid, BS, 0, _s>>>( v, tiles ) ) );
+		};
+		auto by_block = [&]( auto tm ) {
+			if( _lc.recv_block == 256 )
+				go( std::integral_constant<int, 256>{}, tm );
+			else if( _lc.recv_block == 512 )
+				go( std::integral_constant<int, 512>{}, tm );
+			else
+				go( std::integral_constant<int, 1024>{}, tm );
+		};
+		if constexpr( !plastic )
+			if( _lc.tile_w > TILE_MAX )
+			{
+				by_block( std::integral_constant<int, TILE_WIDE>{} );
+				return;
+			}
+		by_block( std::integral_constant<int, TILE_MAX>{} );
 	}
 
 	// Launches `fn` on the net's stream with `start`/`stop` attached to the dispatch itself, so
@@ -1047,14 +1076,7 @@ private:
 	template <bool F>
 	void launch_step_events( hipEvent_t * e, dview const & v )
 	{
-		unsigned const grid = unsigned( _grid_tiles + _grid_plast );
-		int const tiles = _grid_tiles;
-		if( _lc.recv_block == 256 )
-			launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, 256> ), grid, 256, e[0], e[1], v, tiles );
-		else if( _lc.recv_block == 512 )
-			launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, 512> ), grid, 512, e[0], e[1], v, tiles );
-		else
-			launch_timed( reinterpret_cast<void const *>( &k_receive_tiled<Model, F, 1024> ), grid, 1024, e[0], e[1], v, tiles );
+		launch_receive_events<F>( v, e[0], e[1] );
 		launch_update<F>( v, e[2], e[3] );
 	}
 

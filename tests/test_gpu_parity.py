@@ -461,3 +461,29 @@ def test_large_network_update_blocks_bit_exact(gpu, model):
     assert_state_equal(d.neurons(), h.neurons(), "after 300 steps")
     assert d.stats()["spikes"] > 0
     d.close(), h.close()
+
+
+@pytest.mark.parametrize("model,tile_w,block", [("synth", 2048, 256), ("synth", 4096, 1024), ("vogels_abbott", 4096, 256)])
+def test_wide_tiles_bit_exact(gpu, model, tile_w, block):
+    """Delivery tiles wider than 1,024 targets (a second instantiation of the delivery kernel,
+    static models with two kernels per step): fixed point, against the checker, spikes every
+    single step and state bit for bit after graph-replayed steps; plastic models and overlapped
+    steps refuse them."""
+    n = 20_000
+    lay = ([n], [(0, 0, 0.01)])
+    h = O.CpuSnn(model, lay, 1e-4, 1, seed=61, fixed_point=True)
+    d = Snn(model, Layout(*lay), 1e-4, 1, seed=61, fixed_point=True)
+    d.set_launch(tile_w, 0, 0, block, 2)
+    assert d.launch()["tile_w"] == tile_w
+    for i in range(10):
+        np.testing.assert_array_equal(np.sort(d.step(True)), np.sort(h.step(True)), err_msg=f"step {i}")
+    d.run(290)
+    h.run(290)
+    assert_state_equal(d.neurons(), h.neurons(), "after 300 steps")
+    assert d.stats()["events"] > 0
+    d.close(), h.close()
+    for m, delay in (("brunel_3pop_stdp", 1), ("brunel_3pop", 15)):
+        x = Snn(m, Layout(*three_pop_layout(800)), 1e-4, delay, seed=1)
+        with pytest.raises(ValueError):
+            x.set_launch(2048)
+        x.close()
