@@ -297,7 +297,7 @@ int spice_batch_net( spice_batch * batch, int32_t i, spice_net ** out_net );
 
 /* ---- launch shapes and the autotuner --------------------------------------------------- */
 /* No counterpart in the reference (its launch shapes are compile-time constants). tile_w:
- * targets per spike-delivery tile (multiple of 64, 64..1024; up to 4096 for models without
+ * targets per spike-delivery tile (multiple of 64, 64..1024; up to 8192 for models without
  * plasticity that run two kernels per step -- delay 1, or receive() reading target state without
  * a split gate); flush_slots: synapses per
  * plasticity work item (a lazy flush's row slice, or an eager pass's row or target slice;

@@ -836,7 +836,7 @@ int spice_hip_autotune( spice_net * net, int64_t warmup, int64_t steps, spice_la
 			};
 			// delivery: tile width x workgroup size, then the split of each tile's spikes
 			for( int bs : { 256, 512, 1024 } )
-				for( int tw : { 128, 256, 512, 1024, 2048, 4096 } )
+				for( int tw : { 128, 256, 512, 1024, 2048, 4096, 8192 } )
 				{
 					std::int64_t const tiles = ( N + tw - 1 ) / tw;
 					if( ( tw == start.tile_w && bs == start.recv_block ) || N * ( tiles + 1 ) * 4 > ( std::int64_t( 4 ) << 30 ) ||

@@ -458,10 +458,11 @@ constexpr std::int64_t UPDATE_LARGE_N = 131072; // from this many neurons
 #endif
 constexpr int TILE_MAX = 1024; // targets per tile (LDS accumulators)
 // Wide tiles (static models, two-kernel step only: a second instantiation of the delivery kernel,
-// so the other kernels keep the smaller LDS footprint). A sparse network's rows have few targets
-// in each 1,024-neuron tile (the reference's synth network: ~1.6), and every delivery workgroup
-// stages all of the step's spikes: wider tiles stage each spike in fewer workgroups.
-constexpr int TILE_WIDE = 4096;
+// so the other kernels keep the smaller LDS footprint; up to 157 KB of LDS for two 8-byte
+// accumulators and 1,024-thread workgroups). A sparse network's rows have few targets in each
+// 1,024-neuron tile (the reference's synth network: ~1.6), and every delivery workgroup stages
+// all of the step's spikes: wider tiles stage each spike in fewer workgroups.
+constexpr int TILE_WIDE = 8192;
 // Delivery workgroups have 256 or 1024 threads (a launch shape): they stage that many spikes
 // per pass in LDS.
 

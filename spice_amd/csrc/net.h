@@ -793,7 +793,7 @@ private:
 		if( c.recv_parts < 1 || c.recv_parts > 16 ) throw std::invalid_argument( "recv_parts must be in [1, 16]" );
 		int const cus = device_cu_count();
 		if( c.tile_w < 64 || c.tile_w > max_tile_w() || c.tile_w % 64 )
-			throw std::invalid_argument( max_tile_w() > TILE_MAX ? "tile_w must be a multiple of 64 in [64, 4096]"
+			throw std::invalid_argument( max_tile_w() > TILE_MAX ? "tile_w must be a multiple of 64 in [64, 8192]"
 			                                                    : "tile_w must be a multiple of 64 in [64, 1024] (wider tiles: static models with a delay of 1)" );
 		if( c.flush_slots < 64 || c.flush_slots > ( 1 << 20 ) || c.flush_slots % 64 )
 			throw std::invalid_argument( "flush_slots must be a multiple of 64 in [64, 2^20]" );

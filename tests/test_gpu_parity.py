@@ -463,7 +463,8 @@ def test_large_network_update_blocks_bit_exact(gpu, model):
     d.close(), h.close()
 
 
-@pytest.mark.parametrize("model,tile_w,block", [("synth", 2048, 256), ("synth", 4096, 1024), ("vogels_abbott", 4096, 256)])
+@pytest.mark.parametrize("model,tile_w,block", [("synth", 2048, 256), ("synth", 4096, 1024), ("synth", 8192, 512),
+                                                  ("vogels_abbott", 8192, 256)])
 def test_wide_tiles_bit_exact(gpu, model, tile_w, block):
     """Delivery tiles wider than 1,024 targets (a second instantiation of the delivery kernel,
     static models with two kernels per step): fixed point, against the checker, spikes every
