@@ -794,7 +794,7 @@ private:
 		int const cus = device_cu_count();
 		if( c.tile_w < 64 || c.tile_w > max_tile_w() || c.tile_w % 64 )
 			throw std::invalid_argument( max_tile_w() > TILE_MAX ? "tile_w must be a multiple of 64 in [64, 8192]"
-			                                                    : "tile_w must be a multiple of 64 in [64, 1024] (wider tiles: static models with a delay of 1)" );
+			                                                    : "tile_w must be a multiple of 64 in [64, 1024] (wider tiles: models without plasticity that run two kernels per step)" );
 		if( c.flush_slots < 64 || c.flush_slots > ( 1 << 20 ) || c.flush_slots % 64 )
 			throw std::invalid_argument( "flush_slots must be a multiple of 64 in [64, 2^20]" );
 		if( c.flush_grid < 1 || c.flush_grid > 64 * cus ) throw std::invalid_argument( "flush_grid must be in [1, 64 x CUs]" );
