@@ -523,8 +523,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU path is a design note, not an implementation", file=sys.stderr)
+        print(f"bench.py: the simulator runs one network per GPU on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding a network over GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     from spice_amd import Layout, Snn
