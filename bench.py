@@ -570,14 +570,17 @@ def main():
     bytes_per = algorithmic_bytes(model, n, pst, args.profile_steps)
     kern = max(ms, key=lambda k: ms[k])
     # launches of each kernel in the profiled steps: two per step, or (overlapped steps) the
-    # first delivery, one overlapped kernel per step boundary and the last update
-    overlapped = ms["step"] > 0
+    # first delivery, one overlapped kernel per step boundary and the last update, or (fused
+    # steps: no delivery kernel of its own) one step kernel per step
+    fused = ms["step"] > 0 and ms["receive"] == 0
+    overlapped = ms["step"] > 0 and not fused
     launches = ({"receive": 1, "update": 1, "step": args.profile_steps - 1} if overlapped
-                else {"receive": args.profile_steps, "update": args.profile_steps, "step": 1})
+                else {"receive": args.profile_steps, "update": args.profile_steps,
+                      "step": args.profile_steps if fused else 1})
     avg_ms = ms[kern] / launches[kern]
     achieved = bytes_per[kern] / (avg_ms * 1e-3) / 1e9
     kernel_names = {"receive": "k_receive_tiled", "update": "k_step_overlap" if overlapped else "k_update",
-                    "step": "k_step_overlap"}
+                    "step": "k_step_fused" if fused else "k_step_overlap"}
     traffic, traffic_src = measured_traffic(args.workload, kernel_names[kern])
     roofline = {
         "bound": "hbm", "kernel": kernel_names[kern], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

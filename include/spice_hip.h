@@ -272,8 +272,11 @@ int spice_hip_run( spice_net * net, int64_t nsteps );
  * kernel_ms[1] neuron update, kernel_ms[2] 0. Overlapped steps (models whose delivery may run
  * before the target's update, delay >= 2): kernel_ms[0] the first step's delivery, kernel_ms[1]
  * the last step's update, kernel_ms[2] the nsteps - 1 kernels that each run the update of a
- * step beside the delivery of the next. Same results as spice_hip_run. Used for roofline
- * figures. */
+ * step beside the delivery of the next. Fused steps (static models whose steps do not overlap,
+ * launch shapes with one delivery workgroup per tile of at most four workgroup widths, every
+ * tile receiving input): kernel_ms[0] = kernel_ms[1] = 0, kernel_ms[2] the nsteps kernels that
+ * each deliver a step's input to a tile and update the tile's neurons. Same results as
+ * spice_hip_run. Used for roofline figures. */
 int spice_hip_run_profiled( spice_net * net, int64_t nsteps, double * kernel_ms );
 
 /* Waits for all work of the net. */

@@ -10,7 +10,9 @@
 //   acc[a]   receive accumulators of attribute a  [2][N] by step parity: fp32 / int64 (fixed
 //                        point) / int32
 //   sa[a]    attribute a of every synapse         CSR order (plastic models only)
-//   ring     int32[(D+1) * N], counts uint32[D+1] spike ids of the last D+1 steps
+//   ring     int32[(D+2) * N], counts uint32[D+2] spike ids of the last D+2 steps (entry t mod
+//                        (D+2): step T's update clears step T+1's entry while step T's
+//                        delivery may still read step T-D's)
 //   hring    uint64[HWORDS * N]  spike-history ring: bit (t mod H) of word (t mod H)/64 of
 //                        neuron i set iff i spiked at step t (H = 256 steps, plastic models)
 //   hasp     uint8[N]    row s has at least one plastic synapse
@@ -54,6 +56,7 @@ struct ctl_t
 	std::int64_t step; // index of the step being executed
 	float sum;         // Kahan sum of dt over completed steps
 	float c;           // Kahan compensation
+	std::int32_t rslot; // spike ring entry of this step: step mod (D + 2)
 };
 
 // Device counters live in per-block slots: block b of any step kernel adds its block-reduced
@@ -100,6 +103,8 @@ struct dview
 	std::int32_t overlap; // 1: the update of step T runs beside the delivery of T + 1 (it writes
 	                      // the spike log's step offsets then; otherwise the delivery does)
 	std::int32_t upd_slot0; // first statistics slot of the update blocks of an overlapped kernel
+	std::int32_t fused;     // 1: one kernel per step, each tile updated by its delivery workgroup
+	                        // (the update then writes the spike log's step offsets)
 	std::uint32_t flags;
 	std::uint64_t seed;
 	float param[4]; // per-net model parameters (snn_info::param)

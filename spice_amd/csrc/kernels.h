@@ -197,6 +197,17 @@ __device__ inline snn_info info_of( dview const & v )
 }
 __device__ inline float step_dt( dview const & v ) { return v.DT - v.ctl->c; }
 
+// Spike ring entry of step t (t >= -D): D + 2 entries, so that the update of step T clears the
+// entry of step T + 1 (last read by the delivery of step T - 1) while step T's delivery may
+// still read the entry of step T - D.
+__device__ inline int ring_slot( dview const & v, std::int64_t t ) { return int( ( t + v.D + 2 ) % ( v.D + 2 ) ); }
+// The same for step ctl->step + k, -D <= k <= 1, from the clock's entry (no division).
+__device__ inline int ring_at( dview const & v, int k )
+{
+	int const r = v.ctl->rslot + k, R = v.D + 2;
+	return r < 0 ? r + R : r >= R ? r - R : r;
+}
+
 // Adds per-lane counts to statistics slot `slot` (a slot is written by one workgroup at a
 // time): wave reduction, then the waves' totals through LDS, then one plain read-add-write
 // by thread 0. Every thread of the block must call it.
@@ -733,17 +744,16 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 	__shared__ std::uint64_t wsum[BS / 64];
 
 	std::int64_t const T = v.ctl->step + ( AHEAD ? 1 : 0 );
-	int const D1 = v.D + 1;
-	int const slot = int( ( T + 1 ) % D1 ); // ring entry of step T - D
+	int const slot = ring_at( v, ( AHEAD ? 1 : 0 ) - v.D ); // ring entry of step T - D
 	std::int32_t const nall = T < v.D ? 0 : std::int32_t( v.counts[slot] );
 	std::int32_t const s_beg = std::int32_t( std::int64_t( nall ) * part / v.rparts );
 	std::int32_t const nspk = std::int32_t( std::int64_t( nall ) * ( part + 1 ) / v.rparts ) - s_beg;
-	if( !v.overlap && v.slog && work == 0 && threadIdx.x == 0 )
+	if( !v.overlap && !v.fused && v.slog && work == 0 && threadIdx.x == 0 )
 	{
 		// spike log: step T's ids start where step T-1's (final by now) end
 		std::int64_t const k = T - v.slog->t0;
 		if( k >= 0 && k < v.slog->nsteps )
-			v.slog_off[k] = k == 0 ? 0ull : v.slog_off[k - 1] + v.counts[( T - 1 ) % D1];
+			v.slog_off[k] = k == 0 ? 0ull : v.slog_off[k - 1] + v.counts[ring_at( v, ( AHEAD ? 1 : 0 ) - 1 )];
 	}
 	std::int32_t const * spk = v.ring + std::int64_t( slot ) * v.N + s_beg;
 	float const dt = step_dt( v );
@@ -1157,10 +1167,11 @@ __global__ __launch_bounds__( BS ) void k_batch_step_overlap( dview const * view
 
 
 
-// Neuron update of block blk (blockDim.x <= 1024 neurons) at step T; statistics go to slot
-// `slot`.
+// Neuron update of neurons [first, first + min(count, blockDim.x)) (blockDim.x <= 1024) at step
+// T; statistics go to slot `slot`. The range that starts at neuron 0 also clears the next
+// step's counters and writes the next step's clock.
 template <typename Model, bool FIXED>
-__device__ inline void update_block( dview const & v, std::int64_t blk, std::int64_t slot )
+__device__ inline void update_range( dview const & v, std::int64_t first, std::int32_t count, std::int64_t slot )
 {
 	using N = typename Model::neuron;
 	using S = typename Model::synapse;
@@ -1169,11 +1180,10 @@ __device__ inline void update_block( dview const & v, std::int64_t blk, std::int
 	constexpr std::uint32_t rmask = receive_mask_v<N>;
 
 	std::int64_t const T = v.ctl->step;
-	int const D1 = v.D + 1;
-	int const wslot = int( T % D1 );
-	if( blk == 0 && threadIdx.x == 0 )
+	int const wslot = ring_at( v, 0 );
+	if( first == 0 && threadIdx.x == 0 )
 	{
-		v.counts[( T + 1 ) % D1] = 0; // consumed by this step's receive; refilled at T + 1
+		v.counts[ring_at( v, 1 )] = 0; // step T - D - 1's entry (delivered at T - 1); refilled at T + 1
 		if constexpr( plastic ) v.fl_count[( T + 1 ) & 3] = 0; // step T - 1's flush may still read slot (T - 1) & 3
 		if constexpr( eager ) v.po_count[( T + 1 ) & 3] = 0;
 		// next step's clock: exactly kahan_sum::add on the host
@@ -1183,14 +1193,15 @@ __device__ inline void update_block( dview const & v, std::int64_t blk, std::int
 		v.ctl_next->c = ( t - c.sum ) - y;
 		v.ctl_next->sum = t;
 		v.ctl_next->step = c.step + 1;
+		v.ctl_next->rslot = c.rslot + 1 == v.D + 2 ? 0 : c.rslot + 1;
 	}
 	float const dt = step_dt( v );
 	snn_info const info = info_of( v );
-	std::int64_t const i = blk * blockDim.x + threadIdx.x;
+	std::int64_t const i = first + threadIdx.x;
 	bool spiked = false, flush = false, post = false;
 	std::int32_t fl_synced = 0, fl_pend = 0;
 
-	if( i < v.N )
+	if( std::int32_t( threadIdx.x ) < count && i < v.N )
 	{
 		typename N::tuple_t s;
 		load_tuple<N>( s, v.na, i );
@@ -1354,20 +1365,21 @@ __device__ inline void update_block( dview const & v, std::int64_t blk, std::int
 		if( t0 ) v.bstats[slot * NSTAT + ST_SPIKES] += t0;
 		if( v.slog )
 		{
-			// where step T's ids start in the log: written by this step's delivery, or (the
-			// overlapped step, whose delivery of step T ran beside step T - 1's update) here,
-			// from step T - 1's offset and count, both final (delay >= 2: this update does not
-			// clear step T - 1's count)
+			// where step T's ids start in the log: written by this step's delivery kernel, or
+			// (the overlapped step, whose delivery of step T ran beside step T - 1's update; the
+			// fused step, whose delivery may not have written it yet) here, from step T - 1's
+			// offset and count, both final (written by an earlier kernel; this update clears
+			// the count of step T + 1, not of T - 1)
 			std::int64_t const k = T - v.slog->t0;
 			std::uint64_t base = 0;
 			if( k >= 0 && k < v.slog->nsteps )
 			{
-				if( !v.overlap )
+				if( !v.overlap && !v.fused )
 					base = v.slog_off[k];
 				else
 				{
-					base = k == 0 ? 0ull : v.slog_off[k - 1] + v.counts[( T - 1 ) % D1];
-					if( blk == 0 ) v.slog_off[k] = base;
+					base = k == 0 ? 0ull : v.slog_off[k - 1] + v.counts[ring_at( v, -1 )];
+					if( first == 0 ) v.slog_off[k] = base;
 				}
 			}
 			lbase = base;
@@ -1407,10 +1419,42 @@ __device__ inline void update_block( dview const & v, std::int64_t blk, std::int
 	}
 }
 
+// Neuron update of block blk (blockDim.x neurons) at step T.
+template <typename Model, bool FIXED>
+__device__ inline void update_block( dview const & v, std::int64_t blk, std::int64_t slot )
+{
+	update_range<Model, FIXED>( v, blk * blockDim.x, std::int32_t( blockDim.x ), slot );
+}
+
 template <typename Model, bool FIXED, int BS = SPICE_UPDATE_BLOCK>
 __global__ __launch_bounds__( BS ) void k_update( dview v )
 {
 	update_block<Model, FIXED>( v, blockIdx.x, blockIdx.x );
+}
+
+// The fused step (static models whose steps do not overlap, one delivery work item per tile,
+// every tile a delivery tile, tiles of at most FUSE_CHUNKS workgroup widths): one kernel per
+// step. The workgroup that delivers step T's arrivals to a tile holds all of the tile's inputs
+// (its own plain stores, each read back by the thread that stored it), so it goes on to update
+// the tile's neurons at step T. No workgroup waits for another, and no delivery reads what an
+// update of the same step writes (other tiles' neurons, the ring entry of step T, the counters
+// of steps T and T + 1). Several work items per tile would need a device-scope fence per item
+// to hand the sums to the tile's last one (with one L2 per XCD, such a fence writes back and
+// invalidates the XCD's L2): measured 4.5-7x slower per step than two kernels. Update
+// statistics go to slots from upd_slot0 (FUSE_CHUNKS per tile).
+constexpr int FUSE_CHUNKS = 4;
+template <typename Model, bool FIXED, int BS>
+__global__ __launch_bounds__( BS ) void k_step_fused( dview v, std::int32_t grecv, std::int32_t upd_slot0 )
+{
+	for( std::int32_t w = blockIdx.x; w < v.nact; w += grecv )
+	{
+		receive_tile<Model, FIXED, BS, false, TILE_MAX>( v, w );
+		std::int32_t const tile = v.tile_ids[w];
+		std::int32_t const t0 = tile * v.tile_w;
+		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
+		for( std::int32_t c = 0; c < tw; c += BS )
+			update_range<Model, FIXED>( v, t0 + c, tw - c, upd_slot0 + std::int64_t( tile ) * FUSE_CHUNKS + c / BS );
+	}
 }
 
 // Batched neuron update: workgroup b updates block block_local[b] of net block_net[b].
@@ -1706,9 +1750,9 @@ __global__ __launch_bounds__( 256 ) void k_sync_eager( dview v, std::int64_t now
 	std::int64_t const x = uniform64( ( std::int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> 6 );
 	std::int64_t const T = now + 1; // the step whose delivery would apply them
 	if( T < v.D ) return;
-	int const D1 = v.D + 1;
-	if( x >= std::int64_t( v.counts[( T + 1 ) % D1] ) ) return; // spikes of step T - D
-	std::int32_t const r = v.ring[std::int64_t( ( T + 1 ) % D1 ) * v.N + x];
+	int const slot = ring_slot( v, T - v.D ); // spikes of step T - D
+	if( x >= std::int64_t( v.counts[slot] ) ) return;
+	std::int32_t const r = v.ring[std::int64_t( slot ) * v.N + x];
 	if( !v.hasp[r] ) return;
 	snn_info const info = info_of( v );
 	std::int64_t const prev = ( now & 1 ) * std::int64_t( v.N );

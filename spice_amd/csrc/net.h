@@ -142,7 +142,7 @@ public:
 		std::int64_t const t = _step;
 		launch_steps( t, 1 );
 		tick();
-		int const slot = int( t % ( _p.delay + 1 ) );
+		int const slot = int( t % ( _p.delay + 2 ) );
 		if( dspk ) *dspk = _ring.data() + std::int64_t( slot ) * _g.n;
 		if( dcnt ) *dcnt = _counts.data() + slot;
 	}
@@ -160,10 +160,18 @@ public:
 	// stream, and adds the summed kernel durations (ms) to kernel_ms[0..2]: receive (delivery,
 	// with the previous step's plasticity), update, and 0; overlapped steps: the first delivery,
 	// the last update (with the passes of the step before it), and the n - 1 overlapped step
-	// kernels.
+	// kernels; fused steps: 0, 0 and the n step kernels.
 	void run_profiled( std::int64_t n, double * kernel_ms ) override
 	{
 		if( n < 0 ) throw std::invalid_argument( "negative step count" );
+		if( _fused )
+		{
+			if( fixed() )
+				run_profiled_fused<true>( n, kernel_ms );
+			else
+				run_profiled_fused<false>( n, kernel_ms );
+			return;
+		}
 		if( _overlap )
 		{
 			if( fixed() )
@@ -220,7 +228,7 @@ public:
 			*count = 0;
 			return;
 		}
-		int const slot = int( ( _step - 1 ) % ( _p.delay + 1 ) );
+		int const slot = int( ( _step - 1 ) % ( _p.delay + 2 ) );
 		std::uint32_t c = 0;
 		SPICE_HIP( hipMemcpyAsync( &c, _counts.data() + slot, 4, hipMemcpyDeviceToHost, _s ) );
 		SPICE_HIP( hipStreamSynchronize( _s ) );
@@ -254,7 +262,7 @@ public:
 	{
 		if( step < 0 || step > INT_MAX - 2 ) throw std::invalid_argument( "invalid step index" );
 		synchronize();
-		ctl_t ctl{ step, sum, c };
+		ctl_t ctl{ step, sum, c, std::int32_t( step % ( _p.delay + 2 ) ) };
 		SPICE_HIP( hipMemcpy( _ctl.data() + ( step & ( CLOCK_SLOTS - 1 ) ), &ctl, sizeof( ctl ), hipMemcpyHostToDevice ) );
 		SPICE_HIP( hipMemset( _counts.data(), 0, _counts.bytes() ) );
 		if constexpr( plastic )
@@ -432,7 +440,7 @@ public:
 		if( _slog_n == 0 ) return 0;
 		SPICE_HIP( hipMemcpy( off, _slog_off.data(), std::size_t( _slog_n ) * 8, hipMemcpyDeviceToHost ) );
 		std::uint32_t last = 0;
-		int const slot = int( ( _step - 1 ) % ( _p.delay + 1 ) );
+		int const slot = int( ( _step - 1 ) % ( _p.delay + 2 ) );
 		SPICE_HIP( hipMemcpy( &last, _counts.data() + slot, 4, hipMemcpyDeviceToHost ) );
 		std::uint64_t const total = off[_slog_n - 1] + last;
 		off[_slog_n] = total;
@@ -554,6 +562,7 @@ private:
 	int _upd_block = SPICE_UPDATE_BLOCK;
 	int _grid_plast = 0; // plasticity workgroups of the delivery kernel
 	bool _overlap = false; // steps run as k_step_overlap kernels
+	bool _fused = false;   // steps run as k_step_fused kernels (decided by the launch shape)
 	launch_cfg _lc{};
 	dbuf<std::int32_t> _pivots;
 	dbuf<std::int32_t> _tile_ids;
@@ -605,8 +614,8 @@ private:
 			using A = typename S::template type<I>;
 			_sa[I].resize( std::max<std::int64_t>( _g.nnz, 1 ) * sizeof( A ) );
 		} );
-		_ring.resize( std::size_t( _p.delay + 1 ) * n );
-		_counts.resize( _p.delay + 1 );
+		_ring.resize( std::size_t( _p.delay + 2 ) * n );
+		_counts.resize( _p.delay + 2 );
 		SPICE_HIP( hipMemsetAsync( _counts.data(), 0, _counts.bytes(), _s ) );
 		if constexpr( plastic )
 		{
@@ -821,13 +830,20 @@ private:
 		_grid_plast = plastic ? std::max( 1, _grid_flush * 256 / c.recv_block ) : 0; // the same waves as flush_grid
 		_lc = c;
 		v.upd_slot0 = upd_slot0();
+		// one kernel per step (k_step_fused): static models whose steps do not overlap, one
+		// delivery work item per tile, every tile a delivery tile, tiles of at most FUSE_CHUNKS
+		// workgroup widths
+		_fused = !plastic && !_overlap && c.recv_parts == 1 && v.nact == v.ntiles && c.tile_w <= TILE_MAX &&
+		         c.tile_w <= FUSE_CHUNKS * c.recv_block;
+		v.fused = _fused ? 1 : 0;
 		// statistics slots: one per workgroup of the widest step kernel (receive: per work item;
 		// eager plasticity: after those, one per plasticity workgroup or synchronization block;
-		// the overlapped step's update blocks after those), and enough for the update kernel's
-		// blocks
+		// the overlapped step's update blocks, or the fused step's FUSE_CHUNKS per tile, after
+		// those), and enough for the update kernel's blocks
 		std::size_t const slots =
 		    std::size_t( std::max( { _grid_update, _grid_upd, 2 * _grid_flush, 2 * _grid_tiles,
-		                             upd_slot0() + _grid_update } ) ) * NSTAT; // (update blocks of >= 256 neurons)
+		                             upd_slot0() + _grid_update, upd_slot0() + FUSE_CHUNKS * v.ntiles } ) ) *
+		    NSTAT; // (update blocks of >= 256 neurons)
 		if( slots > _bstats.size() )
 		{
 			std::vector<unsigned long long> keep( NSTAT, 0ull );
@@ -898,6 +914,17 @@ private:
 	// next delivery kernel (or a synchronization).
 	void launch_steps( std::int64_t t, std::int64_t len )
 	{
+		if( _fused )
+		{
+			for( std::int64_t k = 0; k < len; k++ )
+			{
+				if( fixed() )
+					launch_fused<true>( _vs[( t + k ) & ( CLOCK_SLOTS - 1 )], nullptr, nullptr );
+				else
+					launch_fused<false>( _vs[( t + k ) & ( CLOCK_SLOTS - 1 )], nullptr, nullptr );
+			}
+			return;
+		}
 		if( !_overlap )
 		{
 			for( std::int64_t k = 0; k < len; k++ ) launch_step( t + k );
@@ -1022,6 +1049,63 @@ private:
 			else
 				SPICE_LAUNCH( ( k_update<Model, F><<<_grid_upd, SPICE_UPDATE_BLOCK, 0, _s>>>( v ) ) );
 		}
+	}
+
+	// the fused step kernel (static models), with HIP events attached to its dispatch when
+	// start is not null
+	template <bool F>
+	void launch_fused( dview const & v, hipEvent_t start, hipEvent_t stop )
+	{
+		if constexpr( !plastic )
+		{
+			std::int32_t const tiles = _grid_tiles, s0 = upd_slot0();
+			auto go = [&]( auto b ) {
+				constexpr int BS = decltype( b )::value;
+				if( start )
+					launch_timed( reinterpret_cast<void const *>( &k_step_fused<Model, F, BS> ), unsigned( tiles ), BS, start, stop, v, tiles, s0 );
+				else
+					SPICE_LAUNCH( ( k_step_fused<Model, F, BS><<<unsigned( tiles ), BS, 0, _s>>>( v, tiles, s0 ) ) );
+			};
+			int const bs = _lc.recv_block;
+			bs == 256 ? go( std::integral_constant<int, 256>{} ) : bs == 512 ? go( std::integral_constant<int, 512>{} ) : go( std::integral_constant<int, 1024>{} );
+		}
+	}
+
+	// run_profiled for fused steps: every step's kernel -> kernel_ms[2]
+	template <bool F>
+	void run_profiled_fused( std::int64_t n, double * kernel_ms )
+	{
+		constexpr int batch = 128;
+		synchronize(); // profiled steps run serially on the main stream
+		std::vector<hipEvent_t> ev( 2 * batch, nullptr );
+		try
+		{
+			for( auto & e : ev ) SPICE_HIP( hipEventCreate( &e ) );
+			for( std::int64_t done = 0; done < n; )
+			{
+				int const k = int( std::min<std::int64_t>( batch, n - done ) );
+				for( int i = 0; i < k; i++ )
+				{
+					launch_fused<F>( _vs[_step & ( CLOCK_SLOTS - 1 )], ev[2 * i], ev[2 * i + 1] );
+					tick();
+				}
+				SPICE_HIP( hipStreamSynchronize( _s ) );
+				for( int i = 0; i < k; i++ )
+				{
+					float ms = 0;
+					SPICE_HIP( hipEventElapsedTime( &ms, ev[2 * i], ev[2 * i + 1] ) );
+					kernel_ms[2] += ms;
+				}
+				done += k;
+			}
+		}
+		catch( ... )
+		{
+			for( auto & e : ev )
+				if( e ) (void)hipEventDestroy( e );
+			throw;
+		}
+		for( auto & e : ev ) (void)hipEventDestroy( e );
 	}
 
 	// delivery workgroups, plus (plastic models) the previous step's plasticity workgroups

@@ -61,7 +61,7 @@ inline std::uint64_t estimate_job_bytes( int model, std::int64_t n, std::int64_t
 		std::uint64_t b = 8 * ( N + 1 ) + 4 * E + 4 * N;                 // CSR (+ generator degrees)
 		b += N * Nn::size_in_bytes + N * Nn::size * ( fixed ? 8 : 4 );    // neurons + accumulators
 		b += E * Ss::size_in_bytes;                                        // synapses
-		b += 4 * N * std::uint64_t( delay + 1 ) + 4 * N * ( ntiles + 1 ); // ring, pivots
+		b += 4 * N * std::uint64_t( delay + 2 ) + 4 * N * ( ntiles + 1 ); // ring, pivots
 		if constexpr( is_plastic_v<M> )
 		{
 			b += 8 * HWORDS * N + 36 * N;

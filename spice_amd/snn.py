@@ -227,7 +227,9 @@ class Snn:
         delivery (with the previous step's plasticity work of plastic models), "update" the
         neuron update, "step" 0. Overlapped steps: "receive" is the first delivery, "update"
         the last update (with the passes of the step before it), "step" the nsteps - 1
-        overlapped kernels (update of step T, delivery of T + 1, plasticity work of T - 1)."""
+        overlapped kernels (update of step T, delivery of T + 1, plasticity work of T - 1).
+        Fused steps (static, delay-1 nets at shapes with one delivery workgroup per tile):
+        "receive" and "update" 0, "step" the nsteps kernels that each deliver and update."""
         ms = (_c.c_double * 3)(0.0, 0.0, 0.0)
         _lib.check(_lib.lib().spice_hip_run_profiled(self._h, int(nsteps), ms))
         return {"receive": ms[0], "update": ms[1], "step": ms[2]}

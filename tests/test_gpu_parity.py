@@ -488,3 +488,39 @@ def test_wide_tiles_bit_exact(gpu, model, tile_w, block):
         with pytest.raises(ValueError):
             x.set_launch(2048)
         x.close()
+
+
+@pytest.mark.parametrize("model,n,p,launch", [
+    ("vogels_abbott", 6000, 0.02, (320, 0, 0, 256, 1)),  # a partial second chunk, a ragged last tile
+    ("synth", 5000, 0.05, (1024, 0, 0, 256, 1)),         # four update chunks per tile
+    ("vogels_abbott", 4000, 0.05, (128, 0, 0, 512, 1)),  # tiles narrower than the workgroup
+    ("synth", 3000, 0.05, (512, 0, 0, 1024, 1)),
+    ("vogels_abbott", 4000, 0.05, (256, 0, 0, 256, 3)),  # several parts per tile: two kernels
+    ("synth", 3000, 0.05, (2048, 0, 0, 256, 1))])        # more than four chunks: two kernels
+def test_fused_step_bit_exact(gpu, model, n, p, launch):
+    """Delay-1 static networks whose tiles all receive input, one delivery workgroup per tile
+    of at most four workgroup widths, run one kernel per step: the workgroup that delivers a
+    tile's inputs updates the tile's neurons. Fixed point, against the checker: spikes every
+    single step, state bit for bit after graph-replayed and profiled steps; the profiled steps
+    report the one step kernel only (the other shapes: two kernels)."""
+    fused = launch[4] == 1 and launch[0] <= 4 * launch[3]
+    lay = ([n], [(0, 0, p)])
+    h = O.CpuSnn(model, lay, 1e-4, 1, seed=71, fixed_point=True)
+    d = Snn(model, Layout(*lay), 1e-4, 1, seed=71, fixed_point=True)
+    d.set_launch(*launch)
+    for i in range(12):
+        np.testing.assert_array_equal(np.sort(d.step(True)), np.sort(h.step(True)), err_msg=f"step {i}")
+    d.run(250)
+    h.run(250)
+    assert_state_equal(d.neurons(), h.neurons(), "after 262 steps")
+    ms = d.run_profiled(38)
+    h.run(38)
+    if fused:
+        assert ms["receive"] == 0 and ms["update"] == 0 and ms["step"] > 0, ms
+    else:
+        assert ms["receive"] > 0 and ms["update"] > 0 and ms["step"] == 0, ms
+    assert_state_equal(d.neurons(), h.neurons(), "after 300 steps")
+    for i in range(3):
+        np.testing.assert_array_equal(np.sort(d.step(True)), np.sort(h.step(True)), err_msg=f"step {300 + i}")
+    assert d.stats()["events"] > 0 and d.stats()["spikes"] > 0
+    d.close(), h.close()
