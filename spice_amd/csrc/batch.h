@@ -89,7 +89,14 @@ public:
 				}
 			}
 			// batch step t runs net step host_step + t: clock slot (host_step + t) mod 4
-			for( int p = 0; p < CLOCK_SLOTS; p++ ) vs[p].push_back( n.view( int( ( n.host_step() + p ) & ( CLOCK_SLOTS - 1 ) ) ) );
+			// (the batch runs two kernels per step, or overlapped ones, never a net's fused step:
+			// its delivery writes the spike log's step offsets, which the update then reads)
+			for( int p = 0; p < CLOCK_SLOTS; p++ )
+			{
+				dview w = n.view( int( ( n.host_step() + p ) & ( CLOCK_SLOTS - 1 ) ) );
+				w.fused = 0;
+				vs[p].push_back( w );
+			}
 		}
 		_ntiles = std::int32_t( tn.size() );
 		_nblocks = std::int32_t( bn.size() );

@@ -1348,21 +1348,7 @@ __device__ inline void update_range( dview const & v, std::int64_t first, std::i
 	__syncthreads();
 	if( threadIdx.x == 0 )
 	{
-		unsigned t0 = 0, t1 = 0, t2 = 0;
-		for( int k = 0; k < int( blockDim.x >> 6 ); k++ )
-		{
-			unsigned const a = wcount[k][0], b = wcount[k][1], c = wcount[k][2];
-			wcount[k][0] = t0;
-			wcount[k][1] = t1;
-			wcount[k][2] = t2;
-			t0 += a;
-			t1 += b;
-			t2 += c;
-		}
-		bbase[0] = t0 ? atomicAdd( &v.counts[wslot], t0 ) : 0u;
-		if constexpr( plastic ) bbase[1] = t1 ? atomicAdd( &v.fl_count[T & 3], t1 ) : 0u;
-		if constexpr( eager ) bbase[2] = t2 ? atomicAdd( &v.po_count[T & 3], t2 ) : 0u;
-		if( t0 ) v.bstats[slot * NSTAT + ST_SPIKES] += t0;
+		// (the spike log's loads go out before the counter atomics: they do not depend on them)
 		if( v.slog )
 		{
 			// where step T's ids start in the log: written by this step's delivery kernel, or
@@ -1384,6 +1370,21 @@ __device__ inline void update_range( dview const & v, std::int64_t first, std::i
 			}
 			lbase = base;
 		}
+		unsigned t0 = 0, t1 = 0, t2 = 0;
+		for( int k = 0; k < int( blockDim.x >> 6 ); k++ )
+		{
+			unsigned const a = wcount[k][0], b = wcount[k][1], c = wcount[k][2];
+			wcount[k][0] = t0;
+			wcount[k][1] = t1;
+			wcount[k][2] = t2;
+			t0 += a;
+			t1 += b;
+			t2 += c;
+		}
+		bbase[0] = t0 ? atomicAdd( &v.counts[wslot], t0 ) : 0u;
+		if constexpr( plastic ) bbase[1] = t1 ? atomicAdd( &v.fl_count[T & 3], t1 ) : 0u;
+		if constexpr( eager ) bbase[2] = t2 ? atomicAdd( &v.po_count[T & 3], t2 ) : 0u;
+		if( t0 ) v.bstats[slot * NSTAT + ST_SPIKES] += t0;
 	}
 	__syncthreads();
 	if( spiked )
