@@ -694,8 +694,10 @@ __device__ inline void deliver_event( A &&... a )
 // LDS with LDS atomics; at the end the tile's sums go to the global accumulators with plain
 // stores: one workgroup owns each tile, so no global atomics are needed. AHEAD: the kernel's
 // clock is step T - 1's (the overlapped step kernel delivers the next step beside the update).
-template <typename Model, bool FIXED, int BS, bool AHEAD = false, int TM = TILE_MAX>
-__device__ inline void receive_tile( dview const & v, std::int32_t work )
+// KEEP (the fused step, one work item per tile): the sums stay in LDS for the same workgroup's
+// update of the tile; the returned pointer addresses them ([attribute][TM], as stored below).
+template <typename Model, bool FIXED, int BS, bool AHEAD = false, int TM = TILE_MAX, bool KEEP = false>
+__device__ inline void const * receive_tile( dview const & v, std::int32_t work )
 {
 	// work item = (active tile, part): the tile's spike list is split into v.rparts parts,
 	// each delivered by its own workgroup (more workgroups in flight for few, large tiles);
@@ -1027,7 +1029,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 		bool const owned = v.rparts == 1;
 		std::int64_t const apar = ( T & 1 ) * std::int64_t( v.N );
 		util::for_n<N::size>( [&]( auto I ) {
-			if constexpr( ( ( rmask >> I ) & 1u ) != 0 )
+			if constexpr( !KEEP && ( ( rmask >> I ) & 1u ) != 0 )
 			{
 				using A = acc_type<typename N::template type<I>, FIXED>;
 				constexpr int a = __builtin_popcount( rmask & ( ( 1u << I ) - 1u ) );
@@ -1048,6 +1050,7 @@ __device__ inline void receive_tile( dview const & v, std::int32_t work )
 		__syncthreads();
 	}
 	block_add_stats<2>( v, work, { ev, pev }, { ST_EVENTS, ST_PLASTIC } );
+	return acc;
 }
 
 template <typename Model, bool FIXED>
@@ -1170,8 +1173,11 @@ __global__ __launch_bounds__( BS ) void k_batch_step_overlap( dview const * view
 // Neuron update of neurons [first, first + min(count, blockDim.x)) (blockDim.x <= 1024) at step
 // T; statistics go to slot `slot`. The range that starts at neuron 0 also clears the next
 // step's counters and writes the next step's clock.
+// lds (the fused step): the inputs are the tile sums a KEEP delivery left in LDS for neurons
+// lds_t0 ..; [attribute][lds_stride words]) instead of the global accumulators, which stay zero.
 template <typename Model, bool FIXED>
-__device__ inline void update_range( dview const & v, std::int64_t first, std::int32_t count, std::int64_t slot )
+__device__ inline void update_range( dview const & v, std::int64_t first, std::int32_t count, std::int64_t slot, void const * lds = nullptr,
+                                     std::int32_t lds_t0 = 0, std::int32_t lds_stride = 0 )
 {
 	using N = typename Model::neuron;
 	using S = typename Model::synapse;
@@ -1244,16 +1250,24 @@ __device__ inline void update_range( dview const & v, std::int64_t first, std::i
 			if constexpr( ( ( rmask >> I ) & 1u ) != 0 )
 			{
 				using A = typename N::template type<I>;
+				using AC = acc_type<A, FIXED>;
+				AC a;
+				if( lds )
+				{
+					constexpr int slotI = __builtin_popcount( rmask & ( ( 1u << I ) - 1u ) );
+					a = reinterpret_cast<AC const *>( static_cast<acc_word<N, FIXED> const *>( lds ) + slotI * lds_stride )[i - lds_t0];
+				}
+				else
+				{
+					a = ld<AC>( v.acc[I], ai );
+					st<AC>( v.acc[I], ai, AC( 0 ) );
+				}
 				if constexpr( std::is_floating_point_v<A> && FIXED )
 				{
-					long long const a = ld<long long>( v.acc[I], ai );
-					st<long long>( v.acc[I], ai, 0 );
 					if( open ) std::get<I>( s ) = std::get<I>( s ) + float( a ) * __builtin_ldexpf( 1.0f, -v.shift );
 				}
 				else
 				{
-					A const a = ld<A>( v.acc[I], ai );
-					st<A>( v.acc[I], ai, A( 0 ) );
 					if( open ) std::get<I>( s ) = A( std::get<I>( s ) + a );
 				}
 			}
@@ -1436,8 +1450,8 @@ __global__ __launch_bounds__( BS ) void k_update( dview v )
 // The fused step (static models whose steps do not overlap, one delivery work item per tile,
 // every tile a delivery tile, tiles of at most FUSE_CHUNKS workgroup widths): one kernel per
 // step. The workgroup that delivers step T's arrivals to a tile holds all of the tile's inputs
-// (its own plain stores, each read back by the thread that stored it), so it goes on to update
-// the tile's neurons at step T. No workgroup waits for another, and no delivery reads what an
+// (its LDS sums, which never go to the global accumulators), so it goes on to update the
+// tile's neurons at step T from them. No workgroup waits for another, and no delivery reads what an
 // update of the same step writes (other tiles' neurons, the ring entry of step T, the counters
 // of steps T and T + 1). Several work items per tile would need a device-scope fence per item
 // to hand the sums to the tile's last one (with one L2 per XCD, such a fence writes back and
@@ -1449,12 +1463,12 @@ __global__ __launch_bounds__( BS ) void k_step_fused( dview v, std::int32_t grec
 {
 	for( std::int32_t w = blockIdx.x; w < v.nact; w += grecv )
 	{
-		receive_tile<Model, FIXED, BS, false, TILE_MAX>( v, w );
+		void const * sums = receive_tile<Model, FIXED, BS, false, TILE_MAX, true>( v, w );
 		std::int32_t const tile = v.tile_ids[w];
 		std::int32_t const t0 = tile * v.tile_w;
 		std::int32_t const tw = ( v.N - t0 ) < v.tile_w ? ( v.N - t0 ) : v.tile_w;
 		for( std::int32_t c = 0; c < tw; c += BS )
-			update_range<Model, FIXED>( v, t0 + c, tw - c, upd_slot0 + std::int64_t( tile ) * FUSE_CHUNKS + c / BS );
+			update_range<Model, FIXED>( v, t0 + c, tw - c, upd_slot0 + std::int64_t( tile ) * FUSE_CHUNKS + c / BS, sums, t0, TILE_MAX );
 	}
 }
 
